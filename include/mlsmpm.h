@@ -1,0 +1,166 @@
+/*
+ * mlsmpm.h -- C ABI of the MI355X (gfx950) MLS-MPM fluid step.
+ *
+ * The reference drives its fluid through two JavaScript host classes whose
+ * methods record WebGPU dispatches; this header is the flat C surface a
+ * Node-API addon (or ctypes, cgo, JNI) binds in their place.  Each entry point
+ * names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every call returns MPM_OK (0) or a negative MPM_ERR_* code; the message
+ *     of the last failure on the calling thread is mpm_last_error().
+ *   - Host arrays passed in are copied before the call returns; host arrays
+ *     passed out are filled before the call returns (the call synchronises
+ *     the scene's stream).  mpm_step only enqueues work.
+ *   - One scene = one HIP stream.  A handle is not thread-safe; distinct
+ *     handles may be driven from distinct threads.
+ *   - Layouts handed out are the reference's renderer layouts: positions and
+ *     velocities as stride-16 vec4 (w = 0), the affine matrix C as 9 floats
+ *     row-major per particle, diffuse particles as 48-byte records.
+ */
+#ifndef MLSMPM_H
+#define MLSMPM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_OK 0
+#define MPM_ERR_INVALID -1     /* bad argument or configuration */
+#define MPM_ERR_RANGE -2       /* particle outside the grid or not finite */
+#define MPM_ERR_CAPACITY -3    /* more particles than the scene holds */
+#define MPM_ERR_HIP -4         /* HIP runtime failure (message has details) */
+#define MPM_ERR_NOMEM -5       /* device memory or HBM budget exhausted */
+#define MPM_ERR_UNSUPPORTED -6 /* option this build does not implement */
+#define MPM_ERR_STATE -7       /* call not valid in the object's state */
+#define MPM_ERR_NOT_FOUND -8   /* unknown id */
+
+/* mpm_config.flags */
+#define MPM_FLAG_KEEP_GRID 1u  /* keep the i32 grids readable after a step
+                                  (separate clear pass instead of the fused one) */
+#define MPM_FLAG_NO_LDS 2u     /* force the global-atomic scatter path */
+#define MPM_FLAG_DENSE_GRID 4u /* update every cell, not only active tiles */
+
+/* Scene configuration.  Physics fields and defaults follow the FluidGPU
+ * constructor (fluid-gpu.js:10-45) and the Params uniform (shaders.js:27-49);
+ * diffuse fields follow DiffuseGPU (diffuse-gpu.js:16-31). */
+typedef struct mpm_config {
+    uint32_t grid_x, grid_y, grid_z; /* grid_X_num.. (fluid-gpu.js:11-14) */
+    uint32_t capacity;               /* particleNum (fluid-gpu.js:16) */
+    uint32_t diffuse_capacity;       /* DiffuseGPU maxCount; 0 = none attached */
+    uint32_t sort_period;            /* re-sort particles by cell every k steps; 0 = never */
+    uint32_t flags;                  /* MPM_FLAG_* */
+    int32_t device;                  /* HIP device ordinal */
+    float dt, gravity, rest_density, stiffness, eos_power, viscosity;
+    float hard_min, hard_max_x, hard_max_y, hard_max_z;
+    float wall_min, wall_stiffness, lookahead_k;
+    float ke_threshold, pressure_threshold, crest_dot_threshold, spawn_rate_k;
+    float lifetime_foam, buoyancy, drag, density_spray_max, density_bubble_min;
+    uint32_t seed; /* DiffuseGPU._seed initial value; advanced before each generate */
+    uint32_t reserved[6];
+} mpm_config;
+
+typedef struct mpm_scene mpm_scene;
+
+/* Version string of this library. */
+const char *mpm_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mpm_last_error(void);
+
+/* Fill cfg with the reference defaults for an explicit grid: DT 0.3,
+ * GRAVITY -0.98, REST_DENSITY 8, STIFFNESS 100, EOS_POWER 1, VISCOSITY 0.01,
+ * HARD_MIN 2, HARD_MAX_* = grid - 2, WALL_MIN 3, WALL_STIFFNESS 1,
+ * LOOKAHEAD_K 2 (fluid-gpu.js:24-45) and the DiffuseGPU tunables
+ * (diffuse-gpu.js:20-28), seed 1, sort_period 8. */
+int mpm_config_default(mpm_config *cfg, uint32_t gx, uint32_t gy, uint32_t gz,
+                       uint32_t capacity);
+
+/* Replaces `new FluidGPU(aspectX, aspectY, aspectZ, particleRadius,
+ * particleNum)` (fluid-gpu.js:10-15): grid = ceil(aspect / (4 r)). */
+int mpm_config_from_aspect(mpm_config *cfg, float aspect_x, float aspect_y, float aspect_z,
+                           float particle_radius, uint32_t capacity);
+
+/* Replaces FluidGPU.initGPU + DiffuseGPU.initGPU (fluid-gpu.js:183-187,
+ * diffuse-gpu.js:39-44): allocates every device buffer of the scene.  The
+ * diffuse pool starts all-dead with free list [0..D-1], top = D
+ * (diffuse-gpu.js:60-68). */
+int mpm_create(const mpm_config *cfg, mpm_scene **out);
+
+/* Frees the scene and its device memory. */
+int mpm_destroy(mpm_scene *s);
+
+/* Replaces fillBlock + the initial buffer upload (fluid-gpu.js:107-128,
+ * 278-281): load n particles.  pos3 is n*3 floats (required), vel3 n*3,
+ * C9 n*9, mass n (NULL = zero, zero, 1.0).  Every coordinate must be finite
+ * and in [1, grid - 1) so the 3x3x3 stencil stays inside the grid; otherwise
+ * MPM_ERR_RANGE and the scene is unchanged. */
+int mpm_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, const float *C9,
+              const float *mass);
+
+/* Replaces FluidGPU.simFrame (fluid-gpu.js:648-684), SUBSTEPS = 1, including
+ * DiffuseGPU.step (diffuse-gpu.js:148-166) when diffuse_capacity > 0:
+ * enqueue nsteps whole steps on the scene's stream. */
+int mpm_step(mpm_scene *s, uint32_t nsteps);
+
+/* Wait for all work enqueued on the scene's stream. */
+int mpm_sync(mpm_scene *s);
+
+/* active_particle_num (fluid-gpu.js:17) and steps taken since mpm_reset. */
+int mpm_get_counts(const mpm_scene *s, uint32_t *n, uint64_t *steps);
+
+/* Read particles in reset order, in the layouts the renderer binds
+ * (particlePosBuffer / particleVelBuffer stride 16, particleAffineBuffer,
+ * particleDensityBuffer: fluid-gpu.js:208-212, fluid-renderer.js:315-316).
+ * Any output may be NULL.  pos4/vel4: n*4 floats, C9: n*9, density: n. */
+int mpm_read_particles(mpm_scene *s, float *pos4, float *vel4, float *C9, float *density);
+
+/* Read the grids as the reference leaves them after a step (cellMassBuffer,
+ * cellMvBuffer interleaved idx*3+c, cellMassF32Buffer, cellMvF32Buffer;
+ * fluid-gpu.js:216-219).  The i32 grids are only available with
+ * MPM_FLAG_KEEP_GRID (MPM_ERR_STATE otherwise).  Any output may be NULL. */
+int mpm_read_grid(mpm_scene *s, int32_t *cell_mass, int32_t *cell_mv3, float *cell_mass_f,
+                  float *cell_v3);
+
+/* Read the diffuse pool: D records of 48 bytes {pos vec4, vel vec4, ptype,
+ * alive, lifetime, pad} (shaders.js:725-732; DiffuseGPU.particleBuffer),
+ * the free-list stack and its top.  Any output may be NULL. */
+int mpm_read_diffuse(mpm_scene *s, void *rec48, uint32_t *free_list, int32_t *top);
+
+/* Overwrite the diffuse pool, free list and top (checkpoint restore, tests). */
+int mpm_write_diffuse(mpm_scene *s, const void *rec48, const uint32_t *free_list, int32_t top);
+
+/* Replaces DiffuseGPU.requestDebugReadback/pollDebug (diffuse-gpu.js:172-188):
+ * alive = maxCount - top. */
+int mpm_diffuse_alive(mpm_scene *s, uint32_t *alive);
+
+/* Replaces assignments to the public tunables of FluidGPU / DiffuseGPU
+ * (fluid-gpu.js:24-45, diffuse-gpu.js:20-28): physics and diffuse fields of
+ * cfg take effect from the next step.  Grid size, capacities and device are
+ * fixed at create (MPM_ERR_INVALID if they differ). */
+int mpm_set_params(mpm_scene *s, const mpm_config *cfg);
+int mpm_get_config(const mpm_scene *s, mpm_config *cfg);
+
+/* Set DiffuseGPU._seed (diffuse-gpu.js:31, 149); the next generate uses seed+1. */
+int mpm_set_seed(mpm_scene *s, uint32_t seed);
+
+/* Replaces FluidGPU._writeWallParams's upload (fluid-gpu.js:337-413): the
+ * WallParams uniform, 4 floats header (count in [0]) + 4 walls x 20 floats
+ * (shaders.js:123-133).  NULL clears all walls. */
+int mpm_set_walls(mpm_scene *s, const float *wallp84);
+
+/* The scene's hipStream_t, for callers that order their own work after it. */
+int mpm_get_stream(mpm_scene *s, void **hip_stream);
+
+/* Per-stage GPU time of the last mpm_step call, in milliseconds, measured
+ * with HIP events on the scene's stream: [p2g_mass, p2g_mom, update, g2p,
+ * diffuse, sort, total]; enable with mpm_enable_timing. */
+int mpm_enable_timing(mpm_scene *s, int enable);
+int mpm_stage_times(mpm_scene *s, float *ms7);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

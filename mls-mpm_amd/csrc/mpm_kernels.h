@@ -1,0 +1,58 @@
+// mpm_kernels.h -- host-side launchers of the MLS-MPM kernels (internal).
+#pragma once
+#include "mpm_device.h"
+
+namespace mpm {
+
+// Device buffers of the diffuse pool and its per-step scratch.
+struct DiffuseBufs {
+  DRec *pool;          // dmax records (DiffuseGPU.particleBuffer layout)
+  uint32_t *free_list; // dmax slot indices (stack)
+  int32_t *top;        // stack top
+  uint32_t *seed;      // last-used DiffuseGPU._seed
+  uint32_t *genbits;   // spawner bitmask by reset-order id, ceil(cap/32) words
+  uint32_t *gensrc;    // slot of each spawner, by id (cap words)
+  uint32_t *killbits;  // killed-slot bitmask, ceil(dmax/32) words
+  uint32_t *prefix;    // scan output, max(ceil(cap/32), ceil(dmax/32)) words
+  uint32_t *scratch;   // scan tile sums
+  uint32_t *total;     // scan total
+};
+
+// Tiled path state (mpm_tiled.hip).
+struct TileBufs {
+  int32_t *boxes;        // per chunk: x0,y0,z0,x1,y1,z1 stencil bounds (inclusive)
+  uint8_t *tile_now;     // per 8x8x8 grid tile: touched by this step's scatter
+  uint8_t *tile_prev;    // per grid tile: holds non-zero velocity from a previous step
+  uint32_t ntiles[3];
+};
+
+// Sort buffers (mpm_tiled.hip).
+struct SortBufs {
+  uint32_t *keys;     // per particle: cell key
+  uint32_t *rank;     // per particle: rank within its key (from the histogram atomics)
+  uint32_t *counts;   // per key
+  uint32_t *offsets;  // per key, exclusive scan of counts
+  uint32_t *scratch;  // scan scratch
+  uint32_t nkeys;
+};
+
+void launch_clear_grid(const GState &g, uint32_t G, hipStream_t st);
+void launch_p2g_mass_global(const Params &P, const PState &S, const GState &g, hipStream_t st);
+void launch_p2g_mom_global(const Params &P, const PState &S, const GState &g, hipStream_t st);
+void launch_update_dense(const Params &P, const GState &g, const Walls &W, bool clear,
+                         hipStream_t st);
+void launch_g2p_global(const Params &P, const PState &S, const GState &g, const Walls &W,
+                       hipStream_t st);
+void launch_diffuse(const Params &P, const PState &S, const GState &g, const Walls &W,
+                    const DiffuseBufs &D, hipStream_t st);
+uint32_t scan_scratch_words(uint32_t n);
+void launch_scan(const uint32_t *in, uint32_t n, bool popc, uint32_t *out, uint32_t *scratch,
+                 uint32_t *total, hipStream_t st);
+void launch_export_particles(const PState &S, uint32_t n, float4 *pos4, float4 *vel4, float *C9,
+                             float *dens, hipStream_t st);
+void launch_export_grid(const GState &g, uint32_t G, int32_t *m, int32_t *mv3, float *mf,
+                        float *v3, hipStream_t st);
+void launch_import_particles(const PState &S, uint32_t n, const float *pos3, const float *vel3,
+                             const float *C9, const float *mass, hipStream_t st);
+
+}  // namespace mpm

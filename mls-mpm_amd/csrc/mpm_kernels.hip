@@ -1,0 +1,799 @@
+// mpm_kernels.hip -- gfx950 kernels of the MLS-MPM fluid step.
+//
+// Stage order per step (fluid-gpu.js:663-681, diffuse-gpu.js:148-166):
+//   [clear] -> P2G mass -> P2G momentum -> grid update -> G2P
+//   -> diffuse generate -> diffuse advect
+// The reference's separate COMPUTE_WEIGHTS and DECODE_MASS dispatches are
+// folded into the kernels that consume them (weights are recomputed from the
+// position, which is cheaper than a 60-byte-per-particle cache; decoded mass
+// is produced where it is read).
+//
+// This file holds the direct forms: one thread per particle (or cell), i32
+// global atomics.  They are the always-correct path and the fallback of the
+// tiled path in mpm_tiled.hip.
+#include "mpm_kernels.h"
+
+namespace mpm {
+
+// ---------------------------------------------------------------------------
+// Grid clear (WGSL_CLEAR, shaders.js:196-210).
+__global__ __launch_bounds__(256) void k_clear_grid(GState g, uint32_t G) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  g.m[c] = 0;
+  g.mx[c] = 0;
+  g.my[c] = 0;
+  g.mz[c] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// P2G mass (WGSL_P2G_MASS, shaders.js:242-274): 27 bounds-checked i32 atomics.
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_p2g_mass_global(Params P, PState S, GState g) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  PW pw;
+  particle_weights(S.at(PX, p), S.at(PY, p), S.at(PZ, p), pw);
+  float pm = UNIT ? 1.0f : S.at(PM, p);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      float wxy[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) wxy[i] = pw.w[0][i] * pw.w[1][j];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        int idx = cell_idx_checked(pw.b[0] + i, pw.b[1] + j, pw.b[2] + k, P);
+        if (idx < 0) continue;
+        float w = wxy[i] * pw.w[2][k];
+        atomicAdd(&g.m[idx], encode(UNIT ? w : w * pm));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Shared body of P2G momentum: density, EOS pressure, stress and the APIC
+// momentum contribution per stencil cell (WGSL_P2G_MOM, shaders.js:286-370).
+// `mass_at(k,j,i)` returns the decoded cell mass of stencil cell (i,j,k);
+// `emit(k,j,i,ex,ey,ez)` receives the three encoded contributions.
+template <bool UNIT, class MassAt, class Emit>
+__device__ __forceinline__ void p2g_mom_particle(const Params &P, const PState &S, uint32_t p,
+                                                 const PW &pw, MassAt mass_at, Emit emit) {
+  float vx = S.at(VX, p), vy = S.at(VY, p), vz = S.at(VZ, p);
+  float m = UNIT ? 1.0f : S.at(PM, p);
+  float a00 = S.at(C00, p), a01 = S.at(C01, p), a02 = S.at(C02, p);
+  float a10 = S.at(C10, p), a11 = S.at(C11, p), a12 = S.at(C12, p);
+  float a20 = S.at(C20, p), a21 = S.at(C21, p), a22 = S.at(C22, p);
+
+  float density = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+        density += ((pw.w[0][i] * pw.w[1][j]) * pw.w[2][k]) * mass_at(k, j, i);
+  S.density[p] = density;
+  if (density <= 1e-8f) return;
+
+  float rest = P.rest_density;
+  float vol = m / fmaxf(density, rest * 0.15f);
+  float pressure = eos_pressure(P, density);
+  float visc = P.viscosity;
+  float s00 = -pressure + visc * (a00 + a00);
+  float s11 = -pressure + visc * (a11 + a11);
+  float s22 = -pressure + visc * (a22 + a22);
+  float s01 = visc * (a01 + a10);
+  float s02 = visc * (a02 + a20);
+  float s12 = visc * (a12 + a21);
+  float coef = ((-vol) * 4.0f) * P.dt;
+
+  float d[3][3];  // d[axis][i] = node - particle offset, (-f-1, -f, 1-f)
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    float f = pw.f[a];
+    d[a][0] = (-f) - 1.0f;
+    d[a][1] = -f;
+    d[a][2] = 1.0f - f;
+  }
+  // Products that depend on one axis only, and the (i,j) partial sums:
+  // Q_x = (a00*dx_i + a01*dy_j) + a02*dz_k keeps the reference association.
+  float qx[3][3], qy[3][3], qz[3][3], sx[3][3], sy[3][3], sz[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      float dxi = d[0][i], dyj = d[1][j];
+      qx[j][i] = a00 * dxi + a01 * dyj;
+      qy[j][i] = a10 * dxi + a11 * dyj;
+      qz[j][i] = a20 * dxi + a21 * dyj;
+      sx[j][i] = s00 * dxi + s01 * dyj;
+      sy[j][i] = s01 * dxi + s11 * dyj;
+      sz[j][i] = s02 * dxi + s12 * dyj;
+    }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float wzk = pw.w[2][k], dzk = d[2][k];
+    float qxz = a02 * dzk, qyz = a12 * dzk, qzz = a22 * dzk;
+    float sxz = s02 * dzk, syz = s12 * dzk, szz = s22 * dzk;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      float wyzk = pw.w[1][j] * wzk;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        float w = pw.w[0][i] * wyzk;
+        float Qx = qx[j][i] + qxz;
+        float Qy = qy[j][i] + qyz;
+        float Qz = qz[j][i] + qzz;
+        float fxf = coef * (sx[j][i] + sxz);
+        float fyf = coef * (sy[j][i] + syz);
+        float fzf = coef * (sz[j][i] + szz);
+        float ux = UNIT ? (vx + Qx) : m * (vx + Qx);
+        float uy = UNIT ? (vy + Qy) : m * (vy + Qy);
+        float uz = UNIT ? (vz + Qz) : m * (vz + Qz);
+        emit(k, j, i, encode(w * (ux + fxf)), encode(w * (uy + fyf)), encode(w * (uz + fzf)));
+      }
+    }
+  }
+}
+
+// P2G momentum with global atomics; decodes the cell mass on the fly.
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_p2g_mom_global(Params P, PState S, GState g) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  PW pw;
+  particle_weights(S.at(PX, p), S.at(PY, p), S.at(PZ, p), pw);
+  const uint32_t gx = P.gx, slice = P.gx * P.gy;
+  uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
+  p2g_mom_particle<UNIT>(
+      P, S, p, pw,
+      [&](int k, int j, int i) { return decode(g.m[base + k * slice + j * gx + i]); },
+      [&](int k, int j, int i, int32_t ex, int32_t ey, int32_t ez) {
+        uint32_t c = base + k * slice + j * gx + i;
+        atomicAdd(&g.mx[c], ex);
+        atomicAdd(&g.my[c], ey);
+        atomicAdd(&g.mz[c], ez);
+      });
+}
+
+// ---------------------------------------------------------------------------
+// Grid update of one cell (WGSL_UPDATE_GRID, shaders.js:443-512).  Returns
+// (vx, vy, vz, decoded mass); the mass lane is what DECODE_MASS produced.
+__device__ __forceinline__ float4 update_cell(const Params &P, const Walls &W, int32_t im,
+                                              int32_t imx, int32_t imy, int32_t imz, int cx,
+                                              int cy, int cz) {
+  float cm = decode(im);
+  if (cm <= 0.0f) return make_float4(0.0f, 0.0f, 0.0f, cm);
+  float vx = decode(imx) / cm;
+  float vy = decode(imy) / cm;
+  float vz = decode(imz) / cm;
+  vy += P.gravity * P.dt;
+  if (cx < 2 || cx > (int)P.gx - 3) vx = 0.0f;
+  if (cy < 2 && vy < 0.0f) vy = 0.0f;
+  if (cy > (int)P.gy - 3 && vy > 0.0f) vy = 0.0f;
+  if (cz < 2 || cz > (int)P.gz - 3) vz = 0.0f;
+  if (cy < 2) {
+    vx *= 0.85f;
+    vz *= 0.9f;
+  }
+  if (P.nwalls) {
+    F3 cellPos = f3((float)cx + 0.5f, (float)cy + 0.5f, (float)cz + 0.5f);
+    F3 wv = f3(vx, vy, vz);
+    for (uint32_t wi = 0; wi < P.nwalls; wi++) {
+      const Wall &w = W.w[wi];
+      F3 lp = wall_to_local(cellPos, w);
+      F3 mn = wall_min3(w), mx = wall_max3(w);
+      if (lp.x < mn.x - 1.0f || lp.y < mn.y - 1.0f || lp.z < mn.z - 1.0f ||
+          lp.x > mx.x + 1.0f || lp.y > mx.y + 1.0f || lp.z > mx.z + 1.0f)
+        continue;
+      F3 wvel = wall_vel_at(cellPos, w);
+      F3 nl;
+      float sdf = wall_sdf_normal(lp, mn, mx, nl);
+      if (sdf < 0.0f) {
+        wv = wvel;
+      } else if (sdf < 1.0f) {
+        F3 nn = wall_to_world_dir(nl, w);
+        F3 vrel = f3(wv.x - wvel.x, wv.y - wvel.y, wv.z - wvel.z);
+        float vn = dot3(vrel, nn);
+        if (vn < 0.0f) wv = f3(wv.x - vn * nn.x, wv.y - vn * nn.y, wv.z - vn * nn.z);
+      }
+    }
+    vx = wv.x;
+    vy = wv.y;
+    vz = wv.z;
+  }
+  return make_float4(vx, vy, vz, cm);
+}
+
+// Dense grid update over every cell; optionally clears the accumulators it
+// consumed (the next step's WGSL_CLEAR, fused).
+__global__ __launch_bounds__(256) void k_update_dense(Params P, GState g, Walls W, int clear) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t G = P.gx * P.gy * P.gz;
+  if (c >= G) return;
+  int32_t im = g.m[c], imx = g.mx[c], imy = g.my[c], imz = g.mz[c];
+  int cx = (int)(c % P.gx);
+  int cy = (int)((c / P.gx) % P.gy);
+  int cz = (int)(c / (P.gx * P.gy));
+  g.v[c] = update_cell(P, W, im, imx, imy, imz, cx, cy, cz);
+  if (clear) {
+    g.m[c] = 0;
+    g.mx[c] = 0;
+    g.my[c] = 0;
+    g.mz[c] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// G2P of one particle (WGSL_G2P, shaders.js:589-704) given the 27 gathered
+// cell velocities through `vel_at(k,j,i)`.  Writes pos, vel, C.
+template <class VelAt>
+__device__ __forceinline__ void g2p_particle(const Params &P, const Walls &W, const PState &S,
+                                             uint32_t p, float px, float py, float pz,
+                                             const PW &pw, VelAt vel_at, float3 &np_out) {
+  float gvx = 0.0f, gvy = 0.0f, gvz = 0.0f;
+  float B00 = 0.0f, B01 = 0.0f, B02 = 0.0f;
+  float B10 = 0.0f, B11 = 0.0f, B12 = 0.0f;
+  float B20 = 0.0f, B21 = 0.0f, B22 = 0.0f;
+  float dxs[3], dys[3], dzs[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    dxs[i] = ((float)(pw.b[0] + i) - px) + 0.5f;
+    dys[i] = ((float)(pw.b[1] + i) - py) + 0.5f;
+    dzs[i] = ((float)(pw.b[2] + i) - pz) + 0.5f;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float wzk = pw.w[2][k], dzk = dzs[k];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      float wyzk = pw.w[1][j] * wzk, dyj = dys[j];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        float w = pw.w[0][i] * wyzk;
+        float dxi = dxs[i];
+        float3 cv = vel_at(k, j, i);
+        float wvx = w * cv.x, wvy = w * cv.y, wvz = w * cv.z;
+        gvx += wvx; gvy += wvy; gvz += wvz;
+        B00 += wvx * dxi; B01 += wvx * dyj; B02 += wvx * dzk;
+        B10 += wvy * dxi; B11 += wvy * dyj; B12 += wvy * dzk;
+        B20 += wvz * dxi; B21 += wvz * dyj; B22 += wvz * dzk;
+      }
+    }
+  }
+  S.at(C00, p) = B00 * 4.0f; S.at(C01, p) = B01 * 4.0f; S.at(C02, p) = B02 * 4.0f;
+  S.at(C10, p) = B10 * 4.0f; S.at(C11, p) = B11 * 4.0f; S.at(C12, p) = B12 * 4.0f;
+  S.at(C20, p) = B20 * 4.0f; S.at(C21, p) = B21 * 4.0f; S.at(C22, p) = B22 * 4.0f;
+
+  const float dt = P.dt, lk = P.lookahead_k;
+  F3 np = f3(px + dt * gvx, py + dt * gvy, pz + dt * gvz);
+  F3 nv = f3(gvx, gvy, gvz);
+
+  // Floor spring, FLOOR_SPRING_BAND = 3 cells (shaders.js:5, 648-654).
+  float floorZoneY = P.hard_min + 3.0f;
+  float xny = np.y + (nv.y * dt) * lk;
+  if (xny < floorZoneY) {
+    float distBelowFloor = floorZoneY - xny;
+    float blend = fminf(fmaxf(distBelowFloor / 3.0f, 0.0f), 1.0f);
+    nv.y += (P.wall_stiffness * blend) * distBelowFloor;
+  }
+
+  for (uint32_t wi = 0; wi < P.nwalls; wi++) {  // moving walls (shaders.js:657-693)
+    const Wall &w = W.w[wi];
+    F3 wmin = wall_min3(w), wmax = wall_max3(w);
+    F3 xnw = f3(np.x + (nv.x * dt) * lk, np.y + (nv.y * dt) * lk, np.z + (nv.z * dt) * lk);
+    F3 lxn = wall_to_local(xnw, w);
+    if (inside_open(lxn, wmin, wmax)) {
+      F3 nl;
+      float sxn = wall_sdf_normal(lxn, wmin, wmax, nl);
+      float dist = -sxn;
+      float blend = fminf(fmaxf(dist / P.wall_min, 0.0f), 1.0f);
+      F3 nn = wall_to_world_dir(nl, w);
+      float s = (P.wall_stiffness * blend) * dist;
+      nv = f3(nv.x + nn.x * s, nv.y + nn.y * s, nv.z + nn.z * s);
+    }
+    F3 lnp = wall_to_local(np, w);
+    if (inside_open(lnp, wmin, wmax)) {
+      F3 nl;
+      float snp = wall_sdf_normal(lnp, wmin, wmax, nl);
+      F3 nn = wall_to_world_dir(nl, w);
+      float push = fmaxf(snp, -0.5f);
+      np = f3(np.x - nn.x * push, np.y - nn.y * push, np.z - nn.z * push);
+      F3 wvel = wall_vel_at(np, w);
+      float vn = dot3(f3(nv.x - wvel.x, nv.y - wvel.y, nv.z - wvel.z), nn);
+      if (vn < 0.0f) nv = f3(nv.x - vn * nn.x, nv.y - vn * nn.y, nv.z - vn * nn.z);
+    }
+  }
+
+  // hard_clamp_axis (shaders.js:103-108, 696-700).
+  if (np.x < P.hard_min) { np.x = P.hard_min; nv.x = 0.0f; }
+  if (np.x > P.hard_max_x) { np.x = P.hard_max_x; nv.x = 0.0f; }
+  if (np.y < P.hard_min) { np.y = P.hard_min; nv.y = 0.0f; }
+  if (np.y > P.hard_max_y) { np.y = P.hard_max_y; nv.y = 0.0f; }
+  if (np.z < P.hard_min) { np.z = P.hard_min; nv.z = 0.0f; }
+  if (np.z > P.hard_max_z) { np.z = P.hard_max_z; nv.z = 0.0f; }
+  S.at(PX, p) = np.x; S.at(PY, p) = np.y; S.at(PZ, p) = np.z;
+  S.at(VX, p) = nv.x; S.at(VY, p) = nv.y; S.at(VZ, p) = nv.z;
+  np_out = make_float3(np.x, np.y, np.z);
+}
+
+__global__ __launch_bounds__(256) void k_g2p_global(Params P, PState S, GState g, Walls W) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  float px = S.at(PX, p), py = S.at(PY, p), pz = S.at(PZ, p);
+  PW pw;
+  particle_weights(px, py, pz, pw);
+  const uint32_t gx = P.gx, slice = P.gx * P.gy;
+  uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
+  float3 np;
+  g2p_particle(P, W, S, p, px, py, pz, pw, [&](int k, int j, int i) {
+    float4 v = g.v[base + k * slice + j * gx + i];
+    return make_float3(v.x, v.y, v.z);
+  }, np);
+}
+
+// ---------------------------------------------------------------------------
+// Diffuse generate (WGSL_DIFFUSE_GENERATE, shaders.js:800-855).
+//
+// The reference pops free-list slots with unordered atomics (try_spawn,
+// shaders.js:791-798).  Here the pops are serialised in particle-index order,
+// one of the orders the reference allows: a decide pass marks spawners in a
+// bitmask indexed by reset-order id, a scan ranks them, and an apply pass
+// gives the r-th spawner free_list[top-1-r] while r < top.
+struct GenEval {
+  float score, speed, density;
+  uint32_t rand_seed;
+};
+
+__device__ __forceinline__ bool gen_eval(const Params &P, const PState &S, const GState &g,
+                                         uint32_t s, uint32_t seed, GenEval &e) {
+  float density = S.density[s];
+  if (density <= 0.0f || density > P.density_bubble_min) return false;
+  float vx = S.at(VX, s), vy = S.at(VY, s), vz = S.at(VZ, s);
+  float speed2 = (vx * vx + vy * vy) + vz * vz;
+  float speed = sqrtf(speed2);
+  float m = P.unit_mass ? 1.0f : S.at(PM, s);
+  float ke = (0.5f * m) * speed2;
+  float pressure = eos_pressure(P, density);
+  float trappedAir = fmaxf(0.0f, pressure - P.pressure_threshold);
+
+  // estimate_normal (shaders.js:766-787) on the decoded cell mass.
+  float px = S.at(PX, s), py = S.at(PY, s), pz = S.at(PZ, s);
+  int gxi = (int)floorf(px), gyi = (int)floorf(py), gzi = (int)floorf(pz);
+  int X = (int)P.gx, Y = (int)P.gy, Z = (int)P.gz;
+  float waveCrest = 0.0f;
+  if (!(gxi < 1 || gxi >= X - 1 || gyi < 1 || gyi >= Y - 1 || gzi < 1 || gzi >= Z - 1)) {
+    uint32_t c = (uint32_t)((gzi * Y + gyi) * X + gxi);
+    float mL = g.v[c - 1].w, mR = g.v[c + 1].w;
+    float mD = g.v[c - X].w, mU = g.v[c + X].w;
+    float mB = g.v[c - X * Y].w, mF = g.v[c + X * Y].w;
+    F3 n = f3(mL - mR, mD - mU, mB - mF);
+    float len = sqrtf(dot3(n, n));
+    if (len >= 1e-6f && speed > 1e-4f) {
+      F3 nn = f3(n.x / len, n.y / len, n.z / len);
+      float dotVal = ((vx / speed) * nn.x + (vy / speed) * nn.y) + (vz / speed) * nn.z;
+      if (dotVal > P.crest_dot_threshold) waveCrest = dotVal * speed;
+    }
+  }
+  float score = (ke + trappedAir * 2.0f) + waveCrest * 1.5f;
+  if (score < P.ke_threshold) return false;
+  uint32_t randSeed = S.id[s] * 9781u + seed;
+  float spawnChance = fminf(1.0f, ((score - P.ke_threshold) * P.spawn_rate_k) * P.dt);
+  if (diffuse_rand(randSeed) > spawnChance) return false;
+  e.score = score;
+  e.speed = speed;
+  e.density = density;
+  e.rand_seed = randSeed;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_gen_decide(Params P, PState S, GState g,
+                                                    const uint32_t *seedp, uint32_t *bits,
+                                                    uint32_t *src) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  GenEval e;
+  if (!gen_eval(P, S, g, s, *seedp + 1u, e)) return;
+  uint32_t id = S.id[s];
+  src[id] = s;
+  atomicOr(&bits[id >> 5], 1u << (id & 31));
+}
+
+__global__ __launch_bounds__(256) void k_gen_apply(Params P, PState S, GState g,
+                                                   const uint32_t *seedp, uint32_t *bits,
+                                                   const uint32_t *prefix, const uint32_t *src,
+                                                   const uint32_t *free_list, const int32_t *topp,
+                                                   DRec *pool, uint32_t nwords) {
+  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwords) return;
+  uint32_t word = bits[w];
+  if (!word) return;
+  bits[w] = 0;
+  int32_t top0 = *topp;
+  uint32_t seed = *seedp + 1u;
+  uint32_t r = prefix[w];
+  while (word) {
+    int b = __ffs(word) - 1;
+    word &= word - 1;
+    uint32_t id = w * 32u + (uint32_t)b;
+    int32_t rank = (int32_t)(r++);
+    if (rank >= top0) continue;
+    int32_t fi = top0 - 1 - rank;
+    if (fi < 0 || (uint32_t)fi >= P.dmax) continue;  // only on an inconsistent pool
+    uint32_t slot = free_list[fi];
+    if (slot >= P.dmax) continue;
+    uint32_t s = src[id];
+    GenEval e;
+    if (!gen_eval(P, S, g, s, seed, e)) continue;  // same inputs: always true here
+    float jitter = 0.3f;
+    float jx = ((diffuse_rand(e.rand_seed + 1u) - 0.5f) * jitter) * e.speed;
+    float jy = ((diffuse_rand(e.rand_seed + 2u) - 0.5f) * jitter) * e.speed;
+    float jz = ((diffuse_rand(e.rand_seed + 3u) - 0.5f) * jitter) * e.speed;
+    uint32_t ptype = (e.density < P.density_spray_max) ? 0u : 1u;
+    float lifetime = (ptype == 1u) ? P.lifetime_foam * fminf(1.0f, e.score / P.ke_threshold)
+                                   : 999.0f;
+    DRec rec;
+    rec.pos = make_float4(S.at(PX, s), S.at(PY, s), S.at(PZ, s), 0.0f);
+    rec.vel = make_float4(S.at(VX, s) + jx, S.at(VY, s) + jy, S.at(VZ, s) + jz, 0.0f);
+    rec.ptype = ptype;
+    rec.alive = 1u;
+    rec.lifetime = lifetime;
+    rec.pad = 0.0f;
+    pool[slot] = rec;
+  }
+}
+
+// top -= min(spawners, top); the step's seed becomes the last-used seed.
+__global__ void k_gen_finish(int32_t *topp, const uint32_t *total, uint32_t *seedp) {
+  int32_t top = *topp;
+  int32_t t = (int32_t)*total;
+  int32_t avail = top > 0 ? top : 0;
+  *topp = top - (t < avail ? t : avail);
+  *seedp = *seedp + 1u;
+}
+
+// ---------------------------------------------------------------------------
+// Diffuse advect (WGSL_DIFFUSE_ADVECT, shaders.js:929-996).  Kills mark a
+// bitmask by slot; a scan then pushes the killed slots in slot order.
+__device__ __forceinline__ F3 sample_grid_velocity(const Params &P, const GState &g, F3 pos) {
+  PW pw;
+  particle_weights(pos.x, pos.y, pos.z, pw);
+  F3 v = f3(0.0f, 0.0f, 0.0f);
+  for (int k = 0; k < 3; k++)
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++) {
+        int idx = cell_idx_checked(pw.b[0] + i, pw.b[1] + j, pw.b[2] + k, P);
+        if (idx < 0) continue;
+        float w = (pw.w[0][i] * pw.w[1][j]) * pw.w[2][k];
+        float4 c = g.v[idx];
+        v.x += w * c.x;
+        v.y += w * c.y;
+        v.z += w * c.z;
+      }
+  return v;
+}
+
+__device__ __forceinline__ void reflect_axis(float &p, float &v, float lo, float hi) {
+  float np = p, nv = v;
+  if (np < lo) {
+    np = lo + (lo - np);
+    nv = (-nv) * 0.3f;
+  } else if (np > hi) {
+    np = hi - (np - hi);
+    nv = (-nv) * 0.3f;
+  }
+  p = fminf(fmaxf(np, lo), hi);
+  v = nv;
+}
+
+__global__ __launch_bounds__(256) void k_advect(Params P, GState g, DRec *pool, uint32_t *killbits,
+                                                Walls W) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.dmax) return;
+  if (pool[i].alive == 0u) return;
+  DRec p = pool[i];
+  const float dt = P.dt, gr = P.gravity;
+  F3 pos3 = f3(p.pos.x, p.pos.y, p.pos.z);
+  bool kill = false;
+  if (p.ptype == 0u) {
+    p.vel.y += gr * dt;
+    p.pos = make_float4(pos3.x + dt * p.vel.x, pos3.y + dt * p.vel.y, pos3.z + dt * p.vel.z, 0.0f);
+  } else if (p.ptype == 1u) {
+    F3 gv = sample_grid_velocity(P, g, pos3);
+    p.pos = make_float4(pos3.x + dt * gv.x, pos3.y + dt * gv.y, pos3.z + dt * gv.z, 0.0f);
+    p.lifetime -= dt;
+    if (p.lifetime <= 0.0f) kill = true;
+  } else if (p.ptype == 2u) {
+    F3 gv = sample_grid_velocity(P, g, pos3);
+    p.vel.x += (dt * P.drag) * (gv.x - p.vel.x);
+    p.vel.y += dt * (P.buoyancy * (-gr) + P.drag * (gv.y - p.vel.y));
+    p.vel.z += (dt * P.drag) * (gv.z - p.vel.z);
+    p.pos = make_float4(pos3.x + dt * p.vel.x, pos3.y + dt * p.vel.y, pos3.z + dt * p.vel.z, 0.0f);
+  }
+  if (!kill) {
+    for (uint32_t wi = 0; wi < P.nwalls; wi++) {
+      const Wall &w = W.w[wi];
+      F3 mn = wall_min3(w), mx = wall_max3(w);
+      F3 lp = wall_to_local(f3(p.pos.x, p.pos.y, p.pos.z), w);
+      if (!inside_open(lp, mn, mx)) continue;
+      F3 nl;
+      float sdf = wall_sdf_normal(lp, mn, mx, nl);
+      F3 nn = wall_to_world_dir(nl, w);
+      p.pos = make_float4(p.pos.x - nn.x * sdf, p.pos.y - nn.y * sdf, p.pos.z - nn.z * sdf, 0.0f);
+      F3 wvel = wall_vel_at(f3(p.pos.x, p.pos.y, p.pos.z), w);
+      F3 vrel = f3(p.vel.x - wvel.x, p.vel.y - wvel.y, p.vel.z - wvel.z);
+      float vn = dot3(vrel, nn);
+      if (vn < 0.0f) {
+        float e = 1.0f + 0.3f;
+        p.vel = make_float4(wvel.x + (vrel.x - (e * vn) * nn.x), wvel.y + (vrel.y - (e * vn) * nn.y),
+                            wvel.z + (vrel.z - (e * vn) * nn.z), 0.0f);
+      }
+    }
+    if (p.pos.y < P.hard_min) kill = true;
+  }
+  if (kill) {  // kill_particle (shaders.js:914-918): payload kept, alive = 0
+    pool[i].alive = 0u;
+    atomicOr(&killbits[i >> 5], 1u << (i & 31));
+    return;
+  }
+  reflect_axis(p.pos.x, p.vel.x, P.hard_min, P.hard_max_x);
+  reflect_axis(p.pos.y, p.vel.y, P.hard_min, P.hard_max_y);
+  reflect_axis(p.pos.z, p.vel.z, P.hard_min, P.hard_max_z);
+  p.pos.w = 0.0f;
+  p.vel.w = 0.0f;
+  pool[i] = p;
+}
+
+__global__ __launch_bounds__(256) void k_kill_push(uint32_t *killbits, const uint32_t *prefix,
+                                                   uint32_t *free_list, const int32_t *topp,
+                                                   uint32_t nwords, uint32_t dmax) {
+  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwords) return;
+  uint32_t word = killbits[w];
+  if (!word) return;
+  killbits[w] = 0;
+  int32_t pos = *topp + (int32_t)prefix[w];
+  while (word) {
+    int b = __ffs(word) - 1;
+    word &= word - 1;
+    if (pos >= 0 && (uint32_t)pos < dmax) free_list[pos] = w * 32u + (uint32_t)b;
+    pos++;
+  }
+}
+
+__global__ void k_kill_finish(int32_t *topp, const uint32_t *total) {
+  *topp = *topp + (int32_t)*total;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of u32 counts (or of the popcounts of bitmask words):
+// tile reduce -> scan of tile sums -> tile scan with carry-in.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+
+template <bool POPC>
+__device__ __forceinline__ uint32_t scan_load(const uint32_t *in, uint32_t n, uint32_t i) {
+  if (i >= n) return 0u;
+  uint32_t v = in[i];
+  return POPC ? (uint32_t)__popc(v) : v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &excl) {
+  __shared__ uint32_t wsum[kScanThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; w++) {
+    uint32_t s = wsum[w];
+    if (w < wid) before += s;
+    total += s;
+  }
+  __syncthreads();
+  excl = before + x - v;
+  return total;
+}
+
+template <bool POPC>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *in, uint32_t n,
+                                                             uint32_t *partials) {
+  uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) s += scan_load<POPC>(in, n, base + i);
+  uint32_t excl;
+  uint32_t total = block_exclusive_scan(s, excl);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// One block scans all tile sums in place (exclusive) and stores the total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(uint32_t *partials, uint32_t nb,
+                                                               uint32_t *total) {
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += kScanTile) {
+    uint32_t idx0 = base + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+      v[i] = (idx0 + i < nb) ? partials[idx0 + i] : 0u;
+      s += v[i];
+    }
+    uint32_t excl;
+    uint32_t tot = block_exclusive_scan(s, excl);
+    uint32_t run = carry + excl;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+      if (idx0 + i < nb) partials[idx0 + i] = run;
+      run += v[i];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <bool POPC>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in, uint32_t n,
+                                                           const uint32_t *partials,
+                                                           uint32_t *out) {
+  uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    v[i] = scan_load<POPC>(in, n, base + i);
+    s += v[i];
+  }
+  uint32_t excl;
+  block_exclusive_scan(s, excl);
+  uint32_t run = partials[blockIdx.x] + excl;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Export to the reference layouts (reset order), for reads.
+__global__ __launch_bounds__(256) void k_export_particles(PState S, uint32_t n, float4 *pos4,
+                                                          float4 *vel4, float *C9, float *dens) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint32_t id = S.id[s];
+  if (pos4) pos4[id] = make_float4(S.at(PX, s), S.at(PY, s), S.at(PZ, s), 0.0f);
+  if (vel4) vel4[id] = make_float4(S.at(VX, s), S.at(VY, s), S.at(VZ, s), 0.0f);
+  if (C9) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) C9[(size_t)id * 9 + k] = S.at(C00 + k, s);
+  }
+  if (dens) dens[id] = S.density[s];
+}
+
+__global__ __launch_bounds__(256) void k_export_grid(GState g, uint32_t G, int32_t *m,
+                                                     int32_t *mv3, float *mf, float *v3) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  if (m) m[c] = g.m[c];
+  if (mv3) {
+    mv3[3 * (size_t)c] = g.mx[c];
+    mv3[3 * (size_t)c + 1] = g.my[c];
+    mv3[3 * (size_t)c + 2] = g.mz[c];
+  }
+  float4 v = g.v[c];
+  if (mf) mf[c] = v.w;
+  if (v3) {
+    v3[3 * (size_t)c] = v.x;
+    v3[3 * (size_t)c + 1] = v.y;
+    v3[3 * (size_t)c + 2] = v.z;
+  }
+}
+
+// Scatter host-order particles into slots (identity order) at reset.
+__global__ __launch_bounds__(256) void k_import_particles(PState S, uint32_t n, const float *pos3,
+                                                          const float *vel3, const float *C9,
+                                                          const float *mass) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  S.at(PX, p) = pos3[3 * (size_t)p];
+  S.at(PY, p) = pos3[3 * (size_t)p + 1];
+  S.at(PZ, p) = pos3[3 * (size_t)p + 2];
+  S.at(VX, p) = vel3 ? vel3[3 * (size_t)p] : 0.0f;
+  S.at(VY, p) = vel3 ? vel3[3 * (size_t)p + 1] : 0.0f;
+  S.at(VZ, p) = vel3 ? vel3[3 * (size_t)p + 2] : 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; k++) S.at(C00 + k, p) = C9 ? C9[(size_t)p * 9 + k] : 0.0f;
+  S.at(PM, p) = mass ? mass[p] : 1.0f;
+  S.density[p] = 0.0f;
+  S.id[p] = p;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+static inline uint32_t blocks(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+
+void launch_clear_grid(const GState &g, uint32_t G, hipStream_t st) {
+  if (G) k_clear_grid<<<blocks(G, 256), 256, 0, st>>>(g, G);
+}
+void launch_p2g_mass_global(const Params &P, const PState &S, const GState &g, hipStream_t st) {
+  if (!P.n) return;
+  if (P.unit_mass) k_p2g_mass_global<true><<<blocks(P.n, 256), 256, 0, st>>>(P, S, g);
+  else k_p2g_mass_global<false><<<blocks(P.n, 256), 256, 0, st>>>(P, S, g);
+}
+void launch_p2g_mom_global(const Params &P, const PState &S, const GState &g, hipStream_t st) {
+  if (!P.n) return;
+  if (P.unit_mass) k_p2g_mom_global<true><<<blocks(P.n, 256), 256, 0, st>>>(P, S, g);
+  else k_p2g_mom_global<false><<<blocks(P.n, 256), 256, 0, st>>>(P, S, g);
+}
+void launch_update_dense(const Params &P, const GState &g, const Walls &W, bool clear,
+                         hipStream_t st) {
+  uint32_t G = P.gx * P.gy * P.gz;
+  k_update_dense<<<blocks(G, 256), 256, 0, st>>>(P, g, W, clear ? 1 : 0);
+}
+void launch_g2p_global(const Params &P, const PState &S, const GState &g, const Walls &W,
+                       hipStream_t st) {
+  if (!P.n) return;
+  k_g2p_global<<<blocks(P.n, 256), 256, 0, st>>>(P, S, g, W);
+}
+
+uint32_t scan_scratch_words(uint32_t n) { return blocks(n, kScanTile) + 1; }
+
+void launch_scan(const uint32_t *in, uint32_t n, bool popc, uint32_t *out, uint32_t *scratch,
+                 uint32_t *total, hipStream_t st) {
+  uint32_t nb = blocks(n, kScanTile);
+  if (nb == 0) nb = 1;
+  if (popc) k_scan_reduce<true><<<nb, kScanThreads, 0, st>>>(in, n, scratch);
+  else k_scan_reduce<false><<<nb, kScanThreads, 0, st>>>(in, n, scratch);
+  k_scan_partials<<<1, kScanThreads, 0, st>>>(scratch, nb, total);
+  if (popc) k_scan_down<true><<<nb, kScanThreads, 0, st>>>(in, n, scratch, out);
+  else k_scan_down<false><<<nb, kScanThreads, 0, st>>>(in, n, scratch, out);
+}
+
+void launch_diffuse(const Params &P, const PState &S, const GState &g, const Walls &W,
+                    const DiffuseBufs &D, hipStream_t st) {
+  if (!P.dmax) return;
+  uint32_t nwg = blocks(P.n, 32);
+  if (P.n) {
+    k_gen_decide<<<blocks(P.n, 256), 256, 0, st>>>(P, S, g, D.seed, D.genbits, D.gensrc);
+    launch_scan(D.genbits, nwg, true, D.prefix, D.scratch, D.total, st);
+    k_gen_apply<<<blocks(nwg, 256), 256, 0, st>>>(P, S, g, D.seed, D.genbits, D.prefix, D.gensrc,
+                                                  D.free_list, D.top, D.pool, nwg);
+  } else {
+    (void)hipMemsetAsync(D.total, 0, sizeof(uint32_t), st);
+  }
+  k_gen_finish<<<1, 1, 0, st>>>(D.top, D.total, D.seed);
+  k_advect<<<blocks(P.dmax, 256), 256, 0, st>>>(P, g, D.pool, D.killbits, W);
+  uint32_t nwk = blocks(P.dmax, 32);
+  launch_scan(D.killbits, nwk, true, D.prefix, D.scratch, D.total, st);
+  k_kill_push<<<blocks(nwk, 256), 256, 0, st>>>(D.killbits, D.prefix, D.free_list, D.top, nwk,
+                                               P.dmax);
+  k_kill_finish<<<1, 1, 0, st>>>(D.top, D.total);
+}
+
+void launch_export_particles(const PState &S, uint32_t n, float4 *pos4, float4 *vel4, float *C9,
+                             float *dens, hipStream_t st) {
+  if (n) k_export_particles<<<blocks(n, 256), 256, 0, st>>>(S, n, pos4, vel4, C9, dens);
+}
+void launch_export_grid(const GState &g, uint32_t G, int32_t *m, int32_t *mv3, float *mf,
+                        float *v3, hipStream_t st) {
+  if (G) k_export_grid<<<blocks(G, 256), 256, 0, st>>>(g, G, m, mv3, mf, v3);
+}
+void launch_import_particles(const PState &S, uint32_t n, const float *pos3, const float *vel3,
+                             const float *C9, const float *mass, hipStream_t st) {
+  if (n) k_import_particles<<<blocks(n, 256), 256, 0, st>>>(S, n, pos3, vel3, C9, mass);
+}
+
+}  // namespace mpm

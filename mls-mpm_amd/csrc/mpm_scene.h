@@ -1,0 +1,56 @@
+// mpm_scene.h -- scene object behind the C ABI (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/mlsmpm.h"
+#include "mpm_kernels.h"
+
+struct mpm_scene {
+  mpm_config cfg{};
+  hipStream_t stream = nullptr;
+  uint32_t n = 0;        // active particles
+  uint32_t G = 0;        // cells
+  uint64_t steps = 0;    // steps since reset
+  int cur = 0;           // which particle buffer is current (sorting ping-pongs)
+  float *planes[2] = {nullptr, nullptr};
+  uint32_t *ids[2] = {nullptr, nullptr};
+  float *density = nullptr;
+  int unit_mass = 1;
+  mpm::GState g{};
+  mpm::DiffuseBufs d{};
+  mpm::TileBufs t{};
+  mpm::SortBufs sort{};
+  mpm::Walls walls{};
+  uint32_t nwalls = 0;
+  uint32_t since_sort = 0;  // steps since the particles were last sorted
+  bool boxes_valid = false; // chunk boxes describe the current positions
+  void *stage = nullptr;    // device staging for reads/writes
+  size_t stage_bytes = 0;
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  float last_ms[7] = {};
+  size_t bytes = 0;         // device bytes held by the scene
+  void *owned[64] = {};     // allocations to free
+  int nowned = 0;
+};
+
+namespace mpm {
+int set_error(int code, const char *msg);
+int hip_fail(hipError_t e, const char *what);
+int validate_config(const mpm_config &c);
+int scene_init(mpm_scene *s, const mpm_config &c);
+void scene_free(mpm_scene *s);
+int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, const float *C9,
+                const float *mass, bool device_ptrs);
+int scene_step(mpm_scene *s, uint32_t nsteps);
+int scene_read_particles(mpm_scene *s, float *pos4, float *vel4, float *C9, float *density);
+int scene_read_grid(mpm_scene *s, int32_t *m, int32_t *mv3, float *mf, float *v3);
+int scene_read_diffuse(mpm_scene *s, void *rec48, uint32_t *free_list, int32_t *top);
+int scene_write_diffuse(mpm_scene *s, const void *rec48, const uint32_t *free_list, int32_t top);
+int scene_set_seed(mpm_scene *s, uint32_t seed);
+int scene_set_walls(mpm_scene *s, const float *wallp);
+int scene_enable_timing(mpm_scene *s, bool on);
+int scene_stage_times(mpm_scene *s, float *ms7);
+Params scene_params(const mpm_scene *s);
+size_t scene_footprint(const mpm_config &c);
+}  // namespace mpm

@@ -1,0 +1,471 @@
+// mpm_scene.hip -- scene lifetime, state upload/download and the per-step
+// launch sequence of FluidGPU.simFrame + DiffuseGPU.step
+// (fluid-gpu.js:648-684, diffuse-gpu.js:148-166).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "mpm_scene.h"
+
+namespace mpm {
+
+#define HIPCHK(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+  } while (0)
+
+static char g_msg[512];
+static int failf(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int failf(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+  va_end(ap);
+  return set_error(code, g_msg);
+}
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+int validate_config(const mpm_config &c) {
+  if (c.grid_x < 5 || c.grid_y < 5 || c.grid_z < 5)
+    return failf(MPM_ERR_INVALID, "grid must be at least 5 cells per axis (got %ux%ux%u)",
+                 c.grid_x, c.grid_y, c.grid_z);
+  uint64_t G = (uint64_t)c.grid_x * c.grid_y * c.grid_z;
+  if (G > (1ull << 30)) return failf(MPM_ERR_INVALID, "grid has %llu cells, limit 2^30",
+                                     (unsigned long long)G);
+  if (c.capacity > (1u << 30))
+    return failf(MPM_ERR_INVALID, "capacity %u exceeds 2^30", c.capacity);
+  if (c.diffuse_capacity > (1u << 30))
+    return failf(MPM_ERR_INVALID, "diffuse_capacity %u exceeds 2^30", c.diffuse_capacity);
+  if (c.eos_power != 1.0f)
+    return failf(MPM_ERR_UNSUPPORTED, "eos_power %g: only 1 (the reference default) is built",
+                 (double)c.eos_power);
+  const float vals[] = {c.dt, c.gravity, c.rest_density, c.stiffness, c.viscosity, c.hard_min,
+                        c.hard_max_x, c.hard_max_y, c.hard_max_z, c.wall_min, c.wall_stiffness,
+                        c.lookahead_k, c.ke_threshold, c.pressure_threshold,
+                        c.crest_dot_threshold, c.spawn_rate_k, c.lifetime_foam, c.buoyancy,
+                        c.drag, c.density_spray_max, c.density_bubble_min};
+  for (float v : vals)
+    if (!std::isfinite(v)) return failf(MPM_ERR_INVALID, "physics constants must be finite");
+  if (!(c.rest_density > 0.0f)) return failf(MPM_ERR_INVALID, "rest_density must be > 0");
+  // Positions are clamped to [hard_min, hard_max] every step; the stencil of
+  // a particle at x touches floor(x)-1 .. floor(x)+1, so the box must lie in
+  // [1, grid-1).
+  if (!(c.hard_min >= 1.0f) || !(c.hard_max_x < (float)c.grid_x - 1.0f) ||
+      !(c.hard_max_y < (float)c.grid_y - 1.0f) || !(c.hard_max_z < (float)c.grid_z - 1.0f) ||
+      !(c.hard_min <= c.hard_max_x && c.hard_min <= c.hard_max_y && c.hard_min <= c.hard_max_z))
+    return failf(MPM_ERR_INVALID,
+                 "hard bounds must satisfy 1 <= hard_min <= hard_max_* < grid_* - 1");
+  return MPM_OK;
+}
+
+// Device bytes a scene of this configuration holds (used for admission).
+size_t scene_footprint(const mpm_config &c) {
+  size_t cap = c.capacity, G = (size_t)c.grid_x * c.grid_y * c.grid_z, D = c.diffuse_capacity;
+  size_t b = 0;
+  b += 2 * cap * (NPLANES * 4 + 4);     // particle planes + ids, double-buffered for sorting
+  b += cap * 4;                         // density
+  b += G * (4 * 4 + 16);                // i32 accumulators + float4 velocity/mass
+  b += cap * 8 + G * 8 + 4096;          // sort keys/ranks, counts/offsets
+  b += cdiv(cap, 1024) * 24 + 2 * cdiv(G, 512);  // chunk boxes, tile flags
+  if (D) b += D * (48 + 4) + cap * 4 + (cap / 32 + D / 32) * 8 + 4096;
+  return b;
+}
+
+static int dalloc(mpm_scene *s, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return failf(MPM_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  if (s->nowned >= (int)(sizeof s->owned / sizeof s->owned[0]))
+    return failf(MPM_ERR_INVALID, "internal: too many allocations");
+  s->owned[s->nowned++] = *p;
+  s->bytes += bytes;
+  return MPM_OK;
+}
+
+#define DALLOC(ptr, bytes)                                          \
+  do {                                                              \
+    int rc_ = dalloc(s, (void **)&(ptr), (size_t)(bytes));          \
+    if (rc_) return rc_;                                            \
+  } while (0)
+
+Params scene_params(const mpm_scene *s) {
+  const mpm_config &c = s->cfg;
+  Params P{};
+  P.gx = c.grid_x;
+  P.gy = c.grid_y;
+  P.gz = c.grid_z;
+  P.n = s->n;
+  P.dt = c.dt;
+  P.gravity = c.gravity;
+  P.rest_density = c.rest_density;
+  P.stiffness = c.stiffness;
+  P.eos_power = c.eos_power;
+  P.viscosity = c.viscosity;
+  P.hard_min = c.hard_min;
+  P.hard_max_x = c.hard_max_x;
+  P.hard_max_y = c.hard_max_y;
+  P.hard_max_z = c.hard_max_z;
+  P.wall_min = c.wall_min;
+  P.wall_stiffness = c.wall_stiffness;
+  P.lookahead_k = c.lookahead_k;
+  P.ke_threshold = c.ke_threshold;
+  P.spawn_rate_k = c.spawn_rate_k;
+  P.density_spray_max = c.density_spray_max;
+  P.density_bubble_min = c.density_bubble_min;
+  P.lifetime_foam = c.lifetime_foam;
+  P.pressure_threshold = c.pressure_threshold;
+  P.crest_dot_threshold = c.crest_dot_threshold;
+  P.buoyancy = c.buoyancy;
+  P.drag = c.drag;
+  P.dmax = c.diffuse_capacity;
+  P.unit_mass = (uint32_t)s->unit_mass;
+  P.nwalls = s->nwalls;
+  return P;
+}
+
+static PState pstate(const mpm_scene *s, int which) {
+  PState S;
+  S.pl = s->planes[which];
+  S.id = s->ids[which];
+  S.density = s->density;
+  S.cap = s->cfg.capacity ? s->cfg.capacity : 1;
+  return S;
+}
+
+static int reset_diffuse(mpm_scene *s) {
+  uint32_t D = s->cfg.diffuse_capacity;
+  if (!D) return MPM_OK;
+  std::vector<uint32_t> fl(D);
+  for (uint32_t i = 0; i < D; i++) fl[i] = i;
+  HIPCHK(hipMemsetAsync(s->d.pool, 0, (size_t)D * sizeof(DRec), s->stream));
+  HIPCHK(hipMemcpyAsync(s->d.free_list, fl.data(), (size_t)D * 4, hipMemcpyHostToDevice,
+                        s->stream));
+  int32_t top = (int32_t)D;
+  HIPCHK(hipMemcpyAsync(s->d.top, &top, 4, hipMemcpyHostToDevice, s->stream));
+  uint32_t seed = s->cfg.seed;
+  HIPCHK(hipMemcpyAsync(s->d.seed, &seed, 4, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int scene_init(mpm_scene *s, const mpm_config &c) {
+  s->cfg = c;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return failf(MPM_ERR_HIP, "no HIP device available (%s)",
+                 e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+  if (c.device < 0 || c.device >= ndev)
+    return failf(MPM_ERR_INVALID, "device %d out of range (%d devices)", c.device, ndev);
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  const size_t cap = c.capacity ? c.capacity : 1;
+  s->G = c.grid_x * c.grid_y * c.grid_z;
+  const size_t G = s->G;
+  for (int b = 0; b < 2; b++) {
+    DALLOC(s->planes[b], cap * NPLANES * 4);
+    DALLOC(s->ids[b], cap * 4);
+  }
+  DALLOC(s->density, cap * 4);
+  DALLOC(s->g.m, G * 4);
+  DALLOC(s->g.mx, G * 4);
+  DALLOC(s->g.my, G * 4);
+  DALLOC(s->g.mz, G * 4);
+  DALLOC(s->g.v, G * 16);
+  HIPCHK(hipMemsetAsync(s->g.m, 0, G * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->g.mx, 0, G * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->g.my, 0, G * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->g.mz, 0, G * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->g.v, 0, G * 16, s->stream));
+  const uint32_t D = c.diffuse_capacity;
+  if (D) {
+    uint32_t nwg = cdiv(cap, 32), nwk = cdiv(D, 32);
+    uint32_t nw = nwg > nwk ? nwg : nwk;
+    DALLOC(s->d.pool, (size_t)D * sizeof(DRec));
+    DALLOC(s->d.free_list, (size_t)D * 4);
+    DALLOC(s->d.top, 4);
+    DALLOC(s->d.seed, 4);
+    DALLOC(s->d.genbits, (size_t)nwg * 4);
+    DALLOC(s->d.gensrc, cap * 4);
+    DALLOC(s->d.killbits, (size_t)nwk * 4);
+    DALLOC(s->d.prefix, (size_t)nw * 4);
+    DALLOC(s->d.scratch, (size_t)scan_scratch_words(nw) * 4);
+    DALLOC(s->d.total, 4);
+    HIPCHK(hipMemsetAsync(s->d.genbits, 0, (size_t)nwg * 4, s->stream));
+    HIPCHK(hipMemsetAsync(s->d.killbits, 0, (size_t)nwk * 4, s->stream));
+  }
+  for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&s->ev[i]));
+  int rc = reset_diffuse(s);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+void scene_free(mpm_scene *s) {
+  if (!s) return;
+  if (s->stream) {
+    (void)hipSetDevice(s->cfg.device);
+    (void)hipStreamSynchronize(s->stream);
+  }
+  for (int i = 0; i < s->nowned; i++) (void)hipFree(s->owned[i]);
+  s->nowned = 0;
+  if (s->stage) (void)hipFree(s->stage);
+  s->stage = nullptr;
+  for (int i = 0; i < 8; i++)
+    if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  s->stream = nullptr;
+}
+
+static int ensure_stage(mpm_scene *s, size_t bytes) {
+  if (s->stage_bytes >= bytes) return MPM_OK;
+  if (s->stage) HIPCHK(hipFree(s->stage));
+  s->stage = nullptr;
+  s->stage_bytes = 0;
+  hipError_t e = hipMalloc(&s->stage, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return failf(MPM_ERR_NOMEM, "staging hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  s->stage_bytes = bytes;
+  return MPM_OK;
+}
+
+int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, const float *C9,
+                const float *mass, bool device_ptrs) {
+  const mpm_config &c = s->cfg;
+  if (n > c.capacity)
+    return failf(MPM_ERR_CAPACITY, "reset of %u particles exceeds capacity %u", n, c.capacity);
+  if (n && !pos3) return failf(MPM_ERR_INVALID, "reset: pos3 is NULL");
+  int unit = 1;
+  if (!device_ptrs) {
+    const float lim[3] = {(float)c.grid_x - 1.0f, (float)c.grid_y - 1.0f, (float)c.grid_z - 1.0f};
+    for (uint64_t i = 0; i < (uint64_t)n * 3; i++) {
+      float x = pos3[i];
+      if (!(x >= 1.0f && x < lim[i % 3]))
+        return failf(MPM_ERR_RANGE,
+                     "particle %llu: coordinate %u = %g outside [1, %g) or not finite",
+                     (unsigned long long)(i / 3), (unsigned)(i % 3), (double)x, (double)lim[i % 3]);
+    }
+    if (vel3)
+      for (uint64_t i = 0; i < (uint64_t)n * 3; i++)
+        if (!std::isfinite(vel3[i]))
+          return failf(MPM_ERR_RANGE, "particle %llu: velocity not finite",
+                       (unsigned long long)(i / 3));
+    if (C9)
+      for (uint64_t i = 0; i < (uint64_t)n * 9; i++)
+        if (!std::isfinite(C9[i]))
+          return failf(MPM_ERR_RANGE, "particle %llu: C not finite", (unsigned long long)(i / 9));
+    if (mass)
+      for (uint64_t i = 0; i < n; i++) {
+        if (!std::isfinite(mass[i]))
+          return failf(MPM_ERR_RANGE, "particle %llu: mass not finite", (unsigned long long)i);
+        if (mass[i] != 1.0f) unit = 0;
+      }
+  }
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->n = n;
+  s->cur = 0;
+  s->steps = 0;
+  s->since_sort = 0;
+  s->boxes_valid = false;
+  s->unit_mass = unit;
+  PState S = pstate(s, 0);
+  if (n) {
+    const float *dp = pos3, *dv = vel3, *dc = C9, *dm = mass;
+    if (!device_ptrs) {
+      size_t need = (size_t)n * 4 * (3 + (vel3 ? 3 : 0) + (C9 ? 9 : 0) + (mass ? 1 : 0));
+      int rc = ensure_stage(s, need);
+      if (rc) return rc;
+      char *st = (char *)s->stage;
+      size_t off = 0;
+      auto put = [&](const float *h, size_t cnt, const float *&d) -> int {
+        if (!h) {
+          d = nullptr;
+          return MPM_OK;
+        }
+        HIPCHK(hipMemcpyAsync(st + off, h, cnt * 4, hipMemcpyHostToDevice, s->stream));
+        d = (const float *)(st + off);
+        off += cnt * 4;
+        return MPM_OK;
+      };
+      int r2 = put(pos3, (size_t)n * 3, dp);
+      if (!r2) r2 = put(vel3, (size_t)n * 3, dv);
+      if (!r2) r2 = put(C9, (size_t)n * 9, dc);
+      if (!r2) r2 = put(mass, (size_t)n, dm);
+      if (r2) return r2;
+    }
+    launch_import_particles(S, n, dp, dv, dc, dm, s->stream);
+    HIPCHK(hipGetLastError());
+  }
+  const size_t G = s->G;
+  HIPCHK(hipMemsetAsync(s->g.m, 0, G * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->g.mx, 0, G * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->g.my, 0, G * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->g.mz, 0, G * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->g.v, 0, G * 16, s->stream));
+  int rc = reset_diffuse(s);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int scene_step(mpm_scene *s, uint32_t nsteps) {
+  HIPCHK(hipSetDevice(s->cfg.device));
+  const bool keep = (s->cfg.flags & MPM_FLAG_KEEP_GRID) != 0;
+  for (uint32_t it = 0; it < nsteps; it++) {
+    Params P = scene_params(s);
+    PState S = pstate(s, s->cur);
+    const bool tm = s->timing && it + 1 == nsteps;
+    hipStream_t st = s->stream;
+    if (tm) HIPCHK(hipEventRecord(s->ev[0], st));
+    if (keep) launch_clear_grid(s->g, s->G, st);
+    launch_p2g_mass_global(P, S, s->g, st);
+    if (tm) HIPCHK(hipEventRecord(s->ev[1], st));
+    launch_p2g_mom_global(P, S, s->g, st);
+    if (tm) HIPCHK(hipEventRecord(s->ev[2], st));
+    launch_update_dense(P, s->g, s->walls, !keep, st);
+    if (tm) HIPCHK(hipEventRecord(s->ev[3], st));
+    launch_g2p_global(P, S, s->g, s->walls, st);
+    if (tm) HIPCHK(hipEventRecord(s->ev[4], st));
+    launch_diffuse(P, S, s->g, s->walls, s->d, st);
+    if (tm) HIPCHK(hipEventRecord(s->ev[5], st));
+    if (tm) HIPCHK(hipEventRecord(s->ev[6], st));
+    HIPCHK(hipGetLastError());
+    s->steps++;
+  }
+  if (s->timing && nsteps) {
+    HIPCHK(hipEventSynchronize(s->ev[6]));
+    for (int i = 0; i < 6; i++) HIPCHK(hipEventElapsedTime(&s->last_ms[i], s->ev[i], s->ev[i + 1]));
+    HIPCHK(hipEventElapsedTime(&s->last_ms[6], s->ev[0], s->ev[6]));
+  }
+  return MPM_OK;
+}
+
+int scene_read_particles(mpm_scene *s, float *pos4, float *vel4, float *C9, float *density) {
+  HIPCHK(hipSetDevice(s->cfg.device));
+  const uint32_t n = s->n;
+  if (!n) return MPM_OK;
+  size_t need = (size_t)n * 4 * (4 + 4 + 9 + 1);
+  int rc = ensure_stage(s, need);
+  if (rc) return rc;
+  char *st = (char *)s->stage;
+  float4 *dpos = pos4 ? (float4 *)st : nullptr;
+  float4 *dvel = vel4 ? (float4 *)(st + (size_t)n * 16) : nullptr;
+  float *dC = C9 ? (float *)(st + (size_t)n * 32) : nullptr;
+  float *dd = density ? (float *)(st + (size_t)n * 68) : nullptr;
+  launch_export_particles(pstate(s, s->cur), n, dpos, dvel, dC, dd, s->stream);
+  HIPCHK(hipGetLastError());
+  if (pos4) HIPCHK(hipMemcpyAsync(pos4, dpos, (size_t)n * 16, hipMemcpyDeviceToHost, s->stream));
+  if (vel4) HIPCHK(hipMemcpyAsync(vel4, dvel, (size_t)n * 16, hipMemcpyDeviceToHost, s->stream));
+  if (C9) HIPCHK(hipMemcpyAsync(C9, dC, (size_t)n * 36, hipMemcpyDeviceToHost, s->stream));
+  if (density)
+    HIPCHK(hipMemcpyAsync(density, dd, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int scene_read_grid(mpm_scene *s, int32_t *m, int32_t *mv3, float *mf, float *v3) {
+  if ((m || mv3) && !(s->cfg.flags & MPM_FLAG_KEEP_GRID))
+    return failf(MPM_ERR_STATE,
+                 "i32 grids are cleared inside the step; create the scene with "
+                 "MPM_FLAG_KEEP_GRID to read them");
+  HIPCHK(hipSetDevice(s->cfg.device));
+  const size_t G = s->G;
+  int rc = ensure_stage(s, G * 4 * 8);
+  if (rc) return rc;
+  char *st = (char *)s->stage;
+  int32_t *dm = m ? (int32_t *)st : nullptr;
+  int32_t *dmv = mv3 ? (int32_t *)(st + G * 4) : nullptr;
+  float *dmf = mf ? (float *)(st + G * 16) : nullptr;
+  float *dv = v3 ? (float *)(st + G * 20) : nullptr;
+  launch_export_grid(s->g, (uint32_t)G, dm, dmv, dmf, dv, s->stream);
+  HIPCHK(hipGetLastError());
+  if (m) HIPCHK(hipMemcpyAsync(m, dm, G * 4, hipMemcpyDeviceToHost, s->stream));
+  if (mv3) HIPCHK(hipMemcpyAsync(mv3, dmv, G * 12, hipMemcpyDeviceToHost, s->stream));
+  if (mf) HIPCHK(hipMemcpyAsync(mf, dmf, G * 4, hipMemcpyDeviceToHost, s->stream));
+  if (v3) HIPCHK(hipMemcpyAsync(v3, dv, G * 12, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int scene_read_diffuse(mpm_scene *s, void *rec48, uint32_t *free_list, int32_t *top) {
+  const uint32_t D = s->cfg.diffuse_capacity;
+  if (!D) return failf(MPM_ERR_STATE, "scene has no diffuse pool (diffuse_capacity = 0)");
+  HIPCHK(hipSetDevice(s->cfg.device));
+  if (rec48)
+    HIPCHK(hipMemcpyAsync(rec48, s->d.pool, (size_t)D * 48, hipMemcpyDeviceToHost, s->stream));
+  if (free_list)
+    HIPCHK(hipMemcpyAsync(free_list, s->d.free_list, (size_t)D * 4, hipMemcpyDeviceToHost,
+                          s->stream));
+  if (top) HIPCHK(hipMemcpyAsync(top, s->d.top, 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int scene_write_diffuse(mpm_scene *s, const void *rec48, const uint32_t *free_list, int32_t top) {
+  const uint32_t D = s->cfg.diffuse_capacity;
+  if (!D) return failf(MPM_ERR_STATE, "scene has no diffuse pool (diffuse_capacity = 0)");
+  if (top < 0 || (uint32_t)top > D)
+    return failf(MPM_ERR_INVALID, "free-list top %d outside [0, %u]", top, D);
+  if (free_list)
+    for (uint32_t i = 0; i < D; i++)
+      if (free_list[i] >= D)
+        return failf(MPM_ERR_INVALID, "free_list[%u] = %u is not a slot index", i, free_list[i]);
+  HIPCHK(hipSetDevice(s->cfg.device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (rec48)
+    HIPCHK(hipMemcpyAsync(s->d.pool, rec48, (size_t)D * 48, hipMemcpyHostToDevice, s->stream));
+  if (free_list)
+    HIPCHK(hipMemcpyAsync(s->d.free_list, free_list, (size_t)D * 4, hipMemcpyHostToDevice,
+                          s->stream));
+  HIPCHK(hipMemcpyAsync(s->d.top, &top, 4, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int scene_set_seed(mpm_scene *s, uint32_t seed) {
+  s->cfg.seed = seed;
+  if (!s->cfg.diffuse_capacity) return MPM_OK;
+  HIPCHK(hipSetDevice(s->cfg.device));
+  HIPCHK(hipMemcpyAsync(s->d.seed, &seed, 4, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int scene_set_walls(mpm_scene *s, const float *wallp) {
+  if (!wallp) {
+    s->nwalls = 0;
+    memset(&s->walls, 0, sizeof s->walls);
+    return MPM_OK;
+  }
+  float c = wallp[0];
+  if (!(c >= 0.0f) || c > 4.0f)
+    return failf(MPM_ERR_INVALID, "wall count %g outside [0, 4]", (double)c);
+  for (int i = 0; i < 4 + 4 * 20; i++)
+    if (!std::isfinite(wallp[i])) return failf(MPM_ERR_INVALID, "wall parameters must be finite");
+  s->nwalls = (uint32_t)c;
+  memcpy(&s->walls, wallp + 4, sizeof s->walls);
+  return MPM_OK;
+}
+
+int scene_enable_timing(mpm_scene *s, bool on) {
+  s->timing = on;
+  return MPM_OK;
+}
+
+int scene_stage_times(mpm_scene *s, float *ms7) {
+  memcpy(ms7, s->last_ms, sizeof s->last_ms);
+  return MPM_OK;
+}
+
+}  // namespace mpm

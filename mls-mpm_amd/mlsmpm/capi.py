@@ -1,0 +1,140 @@
+"""ctypes binding of include/mlsmpm.h (libmlsmpm.so).
+
+The library is the only compute path: there is no CPU fallback.  Loading
+fails loudly when libmlsmpm.so has not been built, and every call that
+returns an MPM_ERR_* code raises MpmError with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from .build import LIB
+
+MPM_OK = 0
+MPM_ERR_INVALID = -1
+MPM_ERR_RANGE = -2
+MPM_ERR_CAPACITY = -3
+MPM_ERR_HIP = -4
+MPM_ERR_NOMEM = -5
+MPM_ERR_UNSUPPORTED = -6
+MPM_ERR_STATE = -7
+MPM_ERR_NOT_FOUND = -8
+
+MPM_FLAG_KEEP_GRID = 1
+MPM_FLAG_NO_LDS = 2
+MPM_FLAG_DENSE_GRID = 4
+
+ERROR_NAMES = {
+    MPM_ERR_INVALID: "MPM_ERR_INVALID", MPM_ERR_RANGE: "MPM_ERR_RANGE",
+    MPM_ERR_CAPACITY: "MPM_ERR_CAPACITY", MPM_ERR_HIP: "MPM_ERR_HIP",
+    MPM_ERR_NOMEM: "MPM_ERR_NOMEM", MPM_ERR_UNSUPPORTED: "MPM_ERR_UNSUPPORTED",
+    MPM_ERR_STATE: "MPM_ERR_STATE", MPM_ERR_NOT_FOUND: "MPM_ERR_NOT_FOUND",
+}
+
+
+class MpmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    """mpm_config (include/mlsmpm.h)."""
+
+    _fields_ = [
+        ("grid_x", C.c_uint32), ("grid_y", C.c_uint32), ("grid_z", C.c_uint32),
+        ("capacity", C.c_uint32), ("diffuse_capacity", C.c_uint32), ("sort_period", C.c_uint32),
+        ("flags", C.c_uint32), ("device", C.c_int32),
+        ("dt", C.c_float), ("gravity", C.c_float), ("rest_density", C.c_float),
+        ("stiffness", C.c_float), ("eos_power", C.c_float), ("viscosity", C.c_float),
+        ("hard_min", C.c_float), ("hard_max_x", C.c_float), ("hard_max_y", C.c_float),
+        ("hard_max_z", C.c_float), ("wall_min", C.c_float), ("wall_stiffness", C.c_float),
+        ("lookahead_k", C.c_float),
+        ("ke_threshold", C.c_float), ("pressure_threshold", C.c_float),
+        ("crest_dot_threshold", C.c_float), ("spawn_rate_k", C.c_float),
+        ("lifetime_foam", C.c_float), ("buoyancy", C.c_float), ("drag", C.c_float),
+        ("density_spray_max", C.c_float), ("density_bubble_min", C.c_float),
+        ("seed", C.c_uint32), ("reserved", C.c_uint32 * 6),
+    ]
+
+    def copy(self) -> "Config":
+        c = Config()
+        C.pointer(c)[0] = self
+        return c
+
+
+# Every entry point declared in include/mlsmpm.h, with its argument types.
+_VP = C.c_void_p
+_SIGNATURES = {
+    "mpm_version": ([], C.c_char_p),
+    "mpm_last_error": ([], C.c_char_p),
+    "mpm_config_default": ([_VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32], C.c_int),
+    "mpm_config_from_aspect": ([_VP, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32], C.c_int),
+    "mpm_create": ([_VP, _VP], C.c_int),
+    "mpm_destroy": ([_VP], C.c_int),
+    "mpm_reset": ([_VP, C.c_uint32, _VP, _VP, _VP, _VP], C.c_int),
+    "mpm_step": ([_VP, C.c_uint32], C.c_int),
+    "mpm_sync": ([_VP], C.c_int),
+    "mpm_get_counts": ([_VP, _VP, _VP], C.c_int),
+    "mpm_read_particles": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
+    "mpm_read_grid": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
+    "mpm_read_diffuse": ([_VP, _VP, _VP, _VP], C.c_int),
+    "mpm_write_diffuse": ([_VP, _VP, _VP, C.c_int32], C.c_int),
+    "mpm_diffuse_alive": ([_VP, _VP], C.c_int),
+    "mpm_set_params": ([_VP, _VP], C.c_int),
+    "mpm_get_config": ([_VP, _VP], C.c_int),
+    "mpm_set_seed": ([_VP, C.c_uint32], C.c_int),
+    "mpm_set_walls": ([_VP, _VP], C.c_int),
+    "mpm_get_stream": ([_VP, _VP], C.c_int),
+    "mpm_enable_timing": ([_VP, C.c_int], C.c_int),
+    "mpm_stage_times": ([_VP, _VP], C.c_int),
+}
+
+_lib = None
+
+
+def lib_path() -> str:
+    return LIB
+
+
+def load() -> C.CDLL:
+    """Load libmlsmpm.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built: run __graft_entry__.build() "
+                               "or python -m mlsmpm.build (there is no CPU fallback)")
+        L = C.CDLL(LIB)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    """Names of the entry points this binding declares."""
+    return list(_SIGNATURES)
+
+
+def check(rc: int) -> int:
+    if rc != MPM_OK:
+        msg = load().mpm_last_error().decode(errors="replace")
+        raise MpmError(rc, msg)
+    return rc
+
+
+def default_config(gx: int, gy: int, gz: int, capacity: int, **kw) -> Config:
+    c = Config()
+    check(load().mpm_config_default(C.byref(c), gx, gy, gz, capacity))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def config_from_aspect(ax: float, ay: float, az: float, radius: float, capacity: int) -> Config:
+    c = Config()
+    check(load().mpm_config_from_aspect(C.byref(c), ax, ay, az, radius, capacity))
+    return c

@@ -1,0 +1,167 @@
+"""Python handle of one scene of libmlsmpm.so (ctypes).
+
+Method names follow the reference host classes: the scene is created from
+a config (FluidGPU / DiffuseGPU constructors, fluid-gpu.js:10-45,
+diffuse-gpu.js:16-31), loaded with reset() (fillBlock + initGPU upload,
+fluid-gpu.js:107-128, 278-281), advanced with step() (simFrame,
+fluid-gpu.js:648-684) and read back in the renderer's layouts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import Config, check, load
+
+# DiffuseParticle, 48 bytes (shaders.js:725-732).
+DREC_DTYPE = np.dtype([("pos", np.float32, 4), ("vel", np.float32, 4), ("ptype", np.uint32),
+                       ("alive", np.uint32), ("lifetime", np.float32), ("pad", np.float32)])
+assert DREC_DTYPE.itemsize == 48
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a, shape):
+    if a is None:
+        return None
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(shape))
+
+
+class Scene:
+    def __init__(self, cfg: Config):
+        L = load()
+        self.cfg = cfg.copy()
+        self._h = C.c_void_p()
+        check(L.mpm_create(C.byref(self.cfg), C.byref(self._h)))
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self):
+        if self._h:
+            load().mpm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- state ---------------------------------------------------------------
+    def reset(self, pos, vel=None, C9=None, mass=None):
+        pos = _f32(pos, (-1, 3))
+        n = pos.shape[0]
+        vel = _f32(vel, (n, 3))
+        C9 = _f32(C9, (n, 9))
+        mass = _f32(mass, (n,))
+        check(load().mpm_reset(self._h, n, _p(pos), _p(vel), _p(C9), _p(mass)))
+
+    def step(self, nsteps: int = 1):
+        check(load().mpm_step(self._h, nsteps))
+
+    def sync(self):
+        check(load().mpm_sync(self._h))
+
+    def counts(self):
+        n = C.c_uint32()
+        st = C.c_uint64()
+        check(load().mpm_get_counts(self._h, C.byref(n), C.byref(st)))
+        return n.value, st.value
+
+    @property
+    def n(self) -> int:
+        return self.counts()[0]
+
+    def read_particles(self, pos=True, vel=True, C9=True, density=True) -> dict:
+        n = self.n
+        out = {
+            "pos4": np.empty((n, 4), np.float32) if pos else None,
+            "vel4": np.empty((n, 4), np.float32) if vel else None,
+            "C9": np.empty((n, 9), np.float32) if C9 else None,
+            "density": np.empty(n, np.float32) if density else None,
+        }
+        check(load().mpm_read_particles(self._h, _p(out["pos4"]), _p(out["vel4"]), _p(out["C9"]),
+                                        _p(out["density"])))
+        return {k: v for k, v in out.items() if v is not None}
+
+    def read_grid(self, ints=True) -> dict:
+        G = self.cfg.grid_x * self.cfg.grid_y * self.cfg.grid_z
+        out = {
+            "cell_mass": np.empty(G, np.int32) if ints else None,
+            "cell_mv": np.empty(3 * G, np.int32) if ints else None,
+            "cell_mass_f": np.empty(G, np.float32),
+            "cell_v": np.empty(3 * G, np.float32),
+        }
+        check(load().mpm_read_grid(self._h, _p(out["cell_mass"]), _p(out["cell_mv"]),
+                                   _p(out["cell_mass_f"]), _p(out["cell_v"])))
+        return {k: v for k, v in out.items() if v is not None}
+
+    def read_diffuse(self):
+        D = self.cfg.diffuse_capacity
+        rec = np.empty(D, DREC_DTYPE)
+        fl = np.empty(D, np.uint32)
+        top = C.c_int32()
+        check(load().mpm_read_diffuse(self._h, _p(rec), _p(fl), C.byref(top)))
+        return rec, fl, top.value
+
+    def write_diffuse(self, rec, free_list, top: int):
+        rec = np.ascontiguousarray(rec, dtype=DREC_DTYPE)
+        fl = np.ascontiguousarray(free_list, dtype=np.uint32)
+        check(load().mpm_write_diffuse(self._h, _p(rec), _p(fl), int(top)))
+
+    def diffuse_alive(self) -> int:
+        a = C.c_uint32()
+        check(load().mpm_diffuse_alive(self._h, C.byref(a)))
+        return a.value
+
+    def set_params(self, **kw):
+        c = self.get_config()
+        for k, v in kw.items():
+            setattr(c, k, v)
+        check(load().mpm_set_params(self._h, C.byref(c)))
+        self.cfg = c
+
+    def get_config(self) -> Config:
+        c = Config()
+        check(load().mpm_get_config(self._h, C.byref(c)))
+        return c
+
+    def set_seed(self, seed: int):
+        check(load().mpm_set_seed(self._h, seed))
+
+    def set_walls(self, wallp):
+        if wallp is None:
+            check(load().mpm_set_walls(self._h, None))
+            return
+        w = np.ascontiguousarray(np.asarray(wallp, dtype=np.float32).reshape(84))
+        check(load().mpm_set_walls(self._h, _p(w)))
+
+    def enable_timing(self, on: bool = True):
+        check(load().mpm_enable_timing(self._h, 1 if on else 0))
+
+    def stage_times(self) -> np.ndarray:
+        ms = np.zeros(7, np.float32)
+        check(load().mpm_stage_times(self._h, _p(ms)))
+        return ms
+
+    def stream(self) -> int:
+        p = C.c_void_p()
+        check(load().mpm_get_stream(self._h, C.byref(p)))
+        return p.value or 0
+
+
+def default_config(gx, gy, gz, capacity, **kw) -> Config:
+    return capi.default_config(gx, gy, gz, capacity, **kw)
