@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""MLS-MPM fluid step throughput on one MI355X.
+
+Workload (default): 3-D dam break on a 256x128x128 grid, 4,199,136 FP32
+particles on a 0.5-cell lattice (x in [2,104), y in [2,43.5), z in [2,126),
+seeded jitter +/-0.1 cell), i32 fixed-point grids, reference physics
+constants, particles re-sorted by cell every 8 steps, no diffuse particles.
+A "step" is one whole MLS-MPM step (P2G mass, P2G momentum, grid update,
+G2P, plus the periodic sort) over all particles, state resident in HBM.
+
+Prints one JSON line:
+  value            particle-steps/s over the timed region (wall clock,
+                   stream-synchronised on both sides)
+  roofline         the dominant kernel's algorithmic bytes per launch over its
+                   mean launch time (HIP events on the scene's stream, every
+                   timed step), against the 8 TB/s HBM peak
+  cpu_baseline     the CPU restatement of the reference step on this host's
+                   cores, on a bounded sample of the same workload
+Extra keys: per-stage mean ms, whole-step roofline fraction, p50/p99 step
+latency.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "mls-mpm_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+# Algorithmic bytes (information moved, 12 B per vec3) per particle and per
+# active cell for each kernel of the step.
+#   P2G mass     : pos 12 + mass 4 per particle; mass write 4 per cell
+#   P2G momentum : pos 12 + vel 12 + C 36 + mass 4 + density 4 per particle;
+#                  mass read 4 + momentum write 12 per cell
+#   grid update  : read mass+momentum 16, write velocity 12, clear 16 per cell
+#   G2P          : pos 12 read, pos 12 + vel 12 + C 36 written per particle;
+#                  velocity read 12 per cell
+ALG_BYTES = {
+    "p2g_mass": (16, 4),
+    "p2g_mom": (68, 16),
+    "update": (0, 44),
+    "g2p": (72, 12),
+}
+STAGES = ["p2g_mass", "p2g_mom", "update", "g2p", "diffuse", "sort", "step"]
+
+
+def workload(name: str, seed: int):
+    from mlsmpm import scenes
+
+    if name == "dam-large":
+        grid, pos, vel = scenes.dam_break_large(seed)
+        desc = "dam-break 256x128x128, column x[2,104) y[2,43.5) z[2,126), 0.5-cell lattice"
+    elif name == "dam-small":
+        grid, pos, vel = scenes.dam_break_small(seed)
+        desc = "dam-break 64^3, block [2,30]x[2,40]x[2,30], 0.5-cell lattice"
+    elif name == "two-block":
+        grid, pos, vel = scenes.two_block_collision(seed)
+        desc = "two-block collision 256^3, blocks at +/-2 cells/step"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return grid, pos, vel, desc
+
+
+def cpu_baseline(grid, pos, vel, seconds: float) -> dict:
+    """The oracle (C restatement of the reference kernels, OpenMP) on a
+    bounded number of steps of the same scene."""
+    from oracle import oracle
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    o = oracle.OracleScene(grid=tuple(grid))
+    o.reset(pos, vel)
+    t0 = time.perf_counter()
+    o.step(1, threads=threads)
+    t1 = time.perf_counter()
+    per = t1 - t0
+    more = max(0, min(20, int(seconds / max(per, 1e-6)) - 1))
+    if more:
+        o.step(more, threads=threads)
+    t2 = time.perf_counter()
+    steps = 1 + more
+    n = len(pos)
+    return {
+        "value": n * steps / (t2 - t0),
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {steps} steps of the same scene ({n} particles), "
+                  f"{threads} OpenMP threads, {t2 - t0:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--workload", default="dam-large", choices=["dam-large", "dam-small", "two-block"])
+    ap.add_argument("--sort-period", type=int, default=8)
+    ap.add_argument("--flags", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; the step is single-device "
+              "(multi-GPU sharding is a design note, not built). Run with --gpus 1.",
+              file=sys.stderr)
+        return 2
+
+    import mlsmpm
+
+    grid, pos, vel, desc = workload(args.workload, args.seed)
+    n = len(pos)
+    cfg = mlsmpm.default_config(*grid, n, sort_period=args.sort_period, flags=args.flags, device=0)
+    s = mlsmpm.Scene(cfg)
+    s.reset(pos, vel)
+    print(f"[bench] {desc}: {n} particles, warmup {args.warmup}", file=sys.stderr, flush=True)
+    s.step(args.warmup)
+    s.sync()
+    g0 = int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"]))
+
+    s.enable_timing(True)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.sync()
+    t1 = time.perf_counter()
+    stage = s.stage_times()
+    step_ms = s.step_times()
+    g1 = int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"]))
+    s.enable_timing(False)
+    elapsed = t1 - t0
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    g_act = 0.5 * (g0 + g1)
+    ms = {k: float(v) for k, v in zip(STAGES, stage)}
+    # dominant kernel and its roofline
+    dom = max(ALG_BYTES, key=lambda k: ms[k])
+    bp, bc = ALG_BYTES[dom]
+    alg = bp * n + bc * g_act
+    achieved = alg / (ms[dom] * 1e-3) if ms[dom] > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        ent = tr.get(args.workload, {}).get(dom)
+        if ent:
+            traffic = ent.get("bytes_per_launch")
+    step_alg = 156 * n + 76 * g_act
+    step_ms_mean = elapsed / args.steps * 1e3
+    line = {
+        "metric": "particle-steps/s + % HBM roofline; batched/served vs plain; p99 step latency",
+        "value": n * args.steps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms_mean,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "particles": n,
+            "grid": list(grid),
+            "grid_dtype": "i32 fixed point (x1e5)",
+            "sort_period": args.sort_period,
+            "diffuse": 0,
+            "active_cells_mean": g_act,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": traffic,
+            "alg_bytes_per_launch": alg,
+            "launch_ms": ms[dom],
+        },
+        "step_roofline_frac": step_alg / (step_ms_mean * 1e-3) / HBM_PEAK,
+        "stage_ms": ms,
+        "step_latency_ms": {
+            "p50": float(np.percentile(step_ms, 50)) if len(step_ms) else None,
+            "p99": float(np.percentile(step_ms, 99)) if len(step_ms) else None,
+        },
+    }
+    s.close()
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(grid, pos, vel, args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

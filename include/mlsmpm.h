@@ -59,7 +59,9 @@ typedef struct mpm_config {
     float ke_threshold, pressure_threshold, crest_dot_threshold, spawn_rate_k;
     float lifetime_foam, buoyancy, drag, density_spray_max, density_bubble_min;
     uint32_t seed; /* DiffuseGPU._seed initial value; advanced before each generate */
-    uint32_t reserved[6];
+    uint32_t tile_slices; /* particles per lane in a P2G chunk (4/8/16/32); 0 = default 16 */
+    uint32_t tile_wmax;   /* LDS window budget in cells (27..4096); 0 = default 3072 */
+    uint32_t reserved[4];
 } mpm_config;
 
 typedef struct mpm_scene mpm_scene;
@@ -154,11 +156,15 @@ int mpm_set_walls(mpm_scene *s, const float *wallp84);
 /* The scene's hipStream_t, for callers that order their own work after it. */
 int mpm_get_stream(mpm_scene *s, void **hip_stream);
 
-/* Per-stage GPU time of the last mpm_step call, in milliseconds, measured
- * with HIP events on the scene's stream: [p2g_mass, p2g_mom, update, g2p,
- * diffuse, sort, total]; enable with mpm_enable_timing. */
+/* Stage timing with HIP events recorded on the scene's stream around every
+ * stage of every step.  mpm_enable_timing(s, 1) clears the record and starts
+ * it; mpm_stage_times gives the mean per step, in milliseconds, of
+ * [p2g_mass, p2g_mom, update, g2p, diffuse, sort, whole step] over the steps
+ * recorded since; mpm_step_times gives each recorded step's total (up to cap
+ * values; *count = number recorded).  Both synchronise. */
 int mpm_enable_timing(mpm_scene *s, int enable);
 int mpm_stage_times(mpm_scene *s, float *ms7);
+int mpm_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count);
 
 #ifdef __cplusplus
 }

@@ -224,4 +224,9 @@ int mpm_stage_times(mpm_scene *s, float *ms7) {
   return scene_stage_times(s, ms7);
 }
 
+int mpm_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_step_times: scene is NULL");
+  return scene_step_times(s, ms, cap, count);
+}
+
 }  // extern "C"

@@ -20,20 +20,18 @@ struct DiffuseBufs {
 
 // Tiled path state (mpm_tiled.hip).
 struct TileBufs {
-  int32_t *boxes;        // per chunk: x0,y0,z0,x1,y1,z1 stencil bounds (inclusive)
-  uint8_t *tile_now;     // per 8x8x8 grid tile: touched by this step's scatter
-  uint8_t *tile_prev;    // per grid tile: holds non-zero velocity from a previous step
-  uint32_t ntiles[3];
+  int32_t *boxes;  // per chunk: window origin x0,y0,z0 and extent wx,wy,wz (wx = 0: fallback)
 };
 
 // Sort buffers (mpm_tiled.hip).
 struct SortBufs {
-  uint32_t *keys;     // per particle: cell key
-  uint32_t *rank;     // per particle: rank within its key (from the histogram atomics)
-  uint32_t *counts;   // per key
-  uint32_t *offsets;  // per key, exclusive scan of counts
-  uint32_t *scratch;  // scan scratch
-  uint32_t nkeys;
+  uint32_t *keys;       // per particle: cell key
+  uint32_t *rank;       // per particle: rank within its key (from the histogram atomics)
+  uint32_t *counts;     // per key
+  uint32_t *offsets;    // per key, exclusive scan of counts
+  uint32_t *scratch;    // scan scratch
+  uint32_t *tile_rank;  // 4x4x4 grid tile -> position in Morton order
+  uint32_t nkeys;       // 64 * number of tiles
 };
 
 void launch_clear_grid(const GState &g, uint32_t G, hipStream_t st);

@@ -2,8 +2,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/mlsmpm.h"
 #include "mpm_kernels.h"
+#include "mpm_tiled.h"
 
 struct mpm_scene {
   mpm_config cfg{};
@@ -23,12 +26,21 @@ struct mpm_scene {
   mpm::Walls walls{};
   uint32_t nwalls = 0;
   uint32_t since_sort = 0;  // steps since the particles were last sorted
-  bool boxes_valid = false; // chunk boxes describe the current positions
+  bool sorted_once = false; // particles sorted at least once since reset
+  uint32_t tdx = 0, tdy = 0, tdz = 0;  // 4x4x4 sort tiles per axis
+  mpm::TileCfg tc{16, 3072};
   void *stage = nullptr;    // device staging for reads/writes
   size_t stage_bytes = 0;
+  // Stage timing: per step, events bracket sort, P2G mass, P2G momentum,
+  // grid update, G2P and diffuse; folded into sums when the pool fills.
   bool timing = false;
-  hipEvent_t ev[8] = {};
-  float last_ms[7] = {};
+  static constexpr int kEvPerStep = 7;
+  static constexpr int kEvSteps = 64;
+  hipEvent_t ev[kEvPerStep * kEvSteps] = {};
+  int ev_used = 0;               // steps recorded in the pool, not yet folded
+  double stage_sum[7] = {};      // ms: p2g_mass, p2g_mom, update, g2p, diffuse, sort, total
+  uint64_t timed_steps = 0;
+  std::vector<float> step_ms;    // per-step totals since timing was enabled
   size_t bytes = 0;         // device bytes held by the scene
   void *owned[64] = {};     // allocations to free
   int nowned = 0;
@@ -51,6 +63,7 @@ int scene_set_seed(mpm_scene *s, uint32_t seed);
 int scene_set_walls(mpm_scene *s, const float *wallp);
 int scene_enable_timing(mpm_scene *s, bool on);
 int scene_stage_times(mpm_scene *s, float *ms7);
+int scene_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count);
 Params scene_params(const mpm_scene *s);
 size_t scene_footprint(const mpm_config &c);
 }  // namespace mpm

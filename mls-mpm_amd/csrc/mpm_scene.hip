@@ -5,6 +5,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "mpm_scene.h"
@@ -40,6 +42,10 @@ int validate_config(const mpm_config &c) {
     return failf(MPM_ERR_INVALID, "capacity %u exceeds 2^30", c.capacity);
   if (c.diffuse_capacity > (1u << 30))
     return failf(MPM_ERR_INVALID, "diffuse_capacity %u exceeds 2^30", c.diffuse_capacity);
+  if (c.tile_slices && !tiled_slices_supported(c.tile_slices))
+    return failf(MPM_ERR_INVALID, "tile_slices %u: supported 4, 8, 16, 32", c.tile_slices);
+  if (c.tile_wmax && (c.tile_wmax < 27 || c.tile_wmax > 4096))
+    return failf(MPM_ERR_INVALID, "tile_wmax %u outside [27, 4096]", c.tile_wmax);
   if (c.eos_power != 1.0f)
     return failf(MPM_ERR_UNSUPPORTED, "eos_power %g: only 1 (the reference default) is built",
                  (double)c.eos_power);
@@ -156,6 +162,30 @@ static int reset_diffuse(mpm_scene *s) {
   return MPM_OK;
 }
 
+// Rank of every 4x4x4 tile when tiles are visited in Morton (Z) order, so
+// that consecutive sort keys stay spatially close across tile boundaries.
+static uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+static std::vector<uint32_t> morton_tile_rank(uint32_t tx, uint32_t ty, uint32_t tz) {
+  std::vector<std::pair<uint64_t, uint32_t>> v;
+  v.reserve((size_t)tx * ty * tz);
+  for (uint32_t z = 0; z < tz; z++)
+    for (uint32_t y = 0; y < ty; y++)
+      for (uint32_t x = 0; x < tx; x++)
+        v.push_back({spread3(x) | (spread3(y) << 1) | (spread3(z) << 2), (z * ty + y) * tx + x});
+  std::sort(v.begin(), v.end());
+  std::vector<uint32_t> rank(v.size());
+  for (size_t i = 0; i < v.size(); i++) rank[v[i].second] = (uint32_t)i;
+  return rank;
+}
+
 int scene_init(mpm_scene *s, const mpm_config &c) {
   s->cfg = c;
   int ndev = 0;
@@ -175,6 +205,25 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     DALLOC(s->ids[b], cap * 4);
   }
   DALLOC(s->density, cap * 4);
+  // tiled P2G and the sort
+  s->tc.slices = c.tile_slices ? c.tile_slices : 16;
+  s->tc.wmax = c.tile_wmax ? c.tile_wmax : 3072;
+  DALLOC(s->t.boxes, (size_t)cdiv(cap, 256) * 6 * 4);
+  s->tdx = cdiv(c.grid_x, 4);
+  s->tdy = cdiv(c.grid_y, 4);
+  s->tdz = cdiv(c.grid_z, 4);
+  {
+    const uint32_t ntiles = s->tdx * s->tdy * s->tdz;
+    s->sort.nkeys = ntiles * 64u;
+    DALLOC(s->sort.keys, cap * 4);
+    DALLOC(s->sort.rank, cap * 4);
+    DALLOC(s->sort.counts, (size_t)s->sort.nkeys * 4);
+    DALLOC(s->sort.offsets, (size_t)s->sort.nkeys * 4);
+    DALLOC(s->sort.scratch, (size_t)scan_scratch_words(s->sort.nkeys) * 4);
+    DALLOC(s->sort.tile_rank, (size_t)ntiles * 4);
+    std::vector<uint32_t> rank = morton_tile_rank(s->tdx, s->tdy, s->tdz);
+    HIPCHK(hipMemcpy(s->sort.tile_rank, rank.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice));
+  }
   DALLOC(s->g.m, G * 4);
   DALLOC(s->g.mx, G * 4);
   DALLOC(s->g.my, G * 4);
@@ -202,7 +251,8 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     HIPCHK(hipMemsetAsync(s->d.genbits, 0, (size_t)nwg * 4, s->stream));
     HIPCHK(hipMemsetAsync(s->d.killbits, 0, (size_t)nwk * 4, s->stream));
   }
-  for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&s->ev[i]));
+  for (int i = 0; i < mpm_scene::kEvPerStep * mpm_scene::kEvSteps; i++)
+    HIPCHK(hipEventCreate(&s->ev[i]));
   int rc = reset_diffuse(s);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -219,7 +269,7 @@ void scene_free(mpm_scene *s) {
   s->nowned = 0;
   if (s->stage) (void)hipFree(s->stage);
   s->stage = nullptr;
-  for (int i = 0; i < 8; i++)
+  for (int i = 0; i < mpm_scene::kEvPerStep * mpm_scene::kEvSteps; i++)
     if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
@@ -277,7 +327,7 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
   s->cur = 0;
   s->steps = 0;
   s->since_sort = 0;
-  s->boxes_valid = false;
+  s->sorted_once = false;
   s->unit_mass = unit;
   PState S = pstate(s, 0);
   if (n) {
@@ -319,34 +369,80 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
   return MPM_OK;
 }
 
+// Fold the recorded per-step events into the stage sums.
+static int fold_timing(mpm_scene *s) {
+  const int E = mpm_scene::kEvPerStep;
+  for (int k = 0; k < s->ev_used; k++) {
+    hipEvent_t *e = &s->ev[k * E];
+    HIPCHK(hipEventSynchronize(e[E - 1]));
+    float ms[6];
+    for (int i = 0; i < 6; i++) HIPCHK(hipEventElapsedTime(&ms[i], e[i], e[i + 1]));
+    float tot;
+    HIPCHK(hipEventElapsedTime(&tot, e[0], e[E - 1]));
+    // event order: start, sort, p2g_mass, p2g_mom, update, g2p, diffuse
+    s->stage_sum[5] += ms[0];
+    s->stage_sum[0] += ms[1];
+    s->stage_sum[1] += ms[2];
+    s->stage_sum[2] += ms[3];
+    s->stage_sum[3] += ms[4];
+    s->stage_sum[4] += ms[5];
+    s->stage_sum[6] += tot;
+    s->step_ms.push_back(tot);
+    s->timed_steps++;
+  }
+  s->ev_used = 0;
+  return MPM_OK;
+}
+
 int scene_step(mpm_scene *s, uint32_t nsteps) {
   HIPCHK(hipSetDevice(s->cfg.device));
   const bool keep = (s->cfg.flags & MPM_FLAG_KEEP_GRID) != 0;
   for (uint32_t it = 0; it < nsteps; it++) {
     Params P = scene_params(s);
     PState S = pstate(s, s->cur);
-    const bool tm = s->timing && it + 1 == nsteps;
     hipStream_t st = s->stream;
-    if (tm) HIPCHK(hipEventRecord(s->ev[0], st));
+    hipEvent_t *e = nullptr;
+    if (s->timing) {
+      if (s->ev_used == mpm_scene::kEvSteps) {
+        int rc = fold_timing(s);
+        if (rc) return rc;
+      }
+      e = &s->ev[s->ev_used * mpm_scene::kEvPerStep];
+      s->ev_used++;
+    }
+    auto mark = [&](int i) -> int {
+      if (e) HIPCHK(hipEventRecord(e[i], st));
+      return MPM_OK;
+    };
+    int rc = mark(0);
+    if (rc) return rc;
+    if (s->cfg.sort_period && s->n &&
+        (!s->sorted_once || s->since_sort >= s->cfg.sort_period)) {
+      PState dst = pstate(s, s->cur ^ 1);
+      launch_sort(P, S, dst, s->sort, s->tdx, s->tdy, s->tc.slices, st);
+      s->cur ^= 1;
+      S = dst;
+      s->since_sort = 0;
+      s->sorted_once = true;
+    }
+    if ((rc = mark(1))) return rc;
+    const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS);
     if (keep) launch_clear_grid(s->g, s->G, st);
-    launch_p2g_mass_global(P, S, s->g, st);
-    if (tm) HIPCHK(hipEventRecord(s->ev[1], st));
-    launch_p2g_mom_global(P, S, s->g, st);
-    if (tm) HIPCHK(hipEventRecord(s->ev[2], st));
+    if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, s->t.boxes, st);
+    else launch_p2g_mass_global(P, S, s->g, st);
+    if ((rc = mark(2))) return rc;
+    if (tiled) launch_p2g_mom_tiled(P, S, s->g, s->tc, s->t.boxes, st);
+    else launch_p2g_mom_global(P, S, s->g, st);
+    if ((rc = mark(3))) return rc;
     launch_update_dense(P, s->g, s->walls, !keep, st);
-    if (tm) HIPCHK(hipEventRecord(s->ev[3], st));
+    if ((rc = mark(4))) return rc;
     launch_g2p_global(P, S, s->g, s->walls, st);
-    if (tm) HIPCHK(hipEventRecord(s->ev[4], st));
+    if ((rc = mark(5))) return rc;
     launch_diffuse(P, S, s->g, s->walls, s->d, st);
-    if (tm) HIPCHK(hipEventRecord(s->ev[5], st));
-    if (tm) HIPCHK(hipEventRecord(s->ev[6], st));
+    if ((rc = mark(6))) return rc;
     HIPCHK(hipGetLastError());
     s->steps++;
-  }
-  if (s->timing && nsteps) {
-    HIPCHK(hipEventSynchronize(s->ev[6]));
-    for (int i = 0; i < 6; i++) HIPCHK(hipEventElapsedTime(&s->last_ms[i], s->ev[i], s->ev[i + 1]));
-    HIPCHK(hipEventElapsedTime(&s->last_ms[6], s->ev[0], s->ev[6]));
+    s->since_sort++;
   }
   return MPM_OK;
 }
@@ -459,12 +555,33 @@ int scene_set_walls(mpm_scene *s, const float *wallp) {
 }
 
 int scene_enable_timing(mpm_scene *s, bool on) {
+  HIPCHK(hipSetDevice(s->cfg.device));
+  HIPCHK(hipStreamSynchronize(s->stream));
   s->timing = on;
+  s->ev_used = 0;
+  for (double &v : s->stage_sum) v = 0.0;
+  s->timed_steps = 0;
+  s->step_ms.clear();
   return MPM_OK;
 }
 
 int scene_stage_times(mpm_scene *s, float *ms7) {
-  memcpy(ms7, s->last_ms, sizeof s->last_ms);
+  HIPCHK(hipSetDevice(s->cfg.device));
+  int rc = fold_timing(s);
+  if (rc) return rc;
+  for (int i = 0; i < 7; i++)
+    ms7[i] = s->timed_steps ? (float)(s->stage_sum[i] / (double)s->timed_steps) : 0.0f;
+  return MPM_OK;
+}
+
+int scene_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count) {
+  HIPCHK(hipSetDevice(s->cfg.device));
+  int rc = fold_timing(s);
+  if (rc) return rc;
+  uint32_t n = (uint32_t)s->step_ms.size();
+  if (count) *count = n;
+  if (ms)
+    for (uint32_t i = 0; i < n && i < cap; i++) ms[i] = s->step_ms[i];
   return MPM_OK;
 }
 

@@ -55,7 +55,8 @@ class Config(C.Structure):
         ("crest_dot_threshold", C.c_float), ("spawn_rate_k", C.c_float),
         ("lifetime_foam", C.c_float), ("buoyancy", C.c_float), ("drag", C.c_float),
         ("density_spray_max", C.c_float), ("density_bubble_min", C.c_float),
-        ("seed", C.c_uint32), ("reserved", C.c_uint32 * 6),
+        ("seed", C.c_uint32), ("tile_slices", C.c_uint32), ("tile_wmax", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
     ]
 
     def copy(self) -> "Config":
@@ -89,6 +90,7 @@ _SIGNATURES = {
     "mpm_get_stream": ([_VP, _VP], C.c_int),
     "mpm_enable_timing": ([_VP, C.c_int], C.c_int),
     "mpm_stage_times": ([_VP, _VP], C.c_int),
+    "mpm_step_times": ([_VP, _VP, C.c_uint32, _VP], C.c_int),
 }
 
 _lib = None

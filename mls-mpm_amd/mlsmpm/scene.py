@@ -157,6 +157,13 @@ class Scene:
         check(load().mpm_stage_times(self._h, _p(ms)))
         return ms
 
+    def step_times(self) -> np.ndarray:
+        cnt = C.c_uint32()
+        check(load().mpm_step_times(self._h, None, 0, C.byref(cnt)))
+        ms = np.zeros(max(cnt.value, 1), np.float32)
+        check(load().mpm_step_times(self._h, _p(ms), cnt.value, C.byref(cnt)))
+        return ms[: cnt.value]
+
     def stream(self) -> int:
         p = C.c_void_p()
         check(load().mpm_get_stream(self._h, C.byref(p)))

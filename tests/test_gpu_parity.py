@@ -17,11 +17,29 @@ from oracle import oracle
 pytestmark = pytest.mark.gpu
 
 KEEP = mlsmpm.MPM_FLAG_KEEP_GRID
+NO_LDS = mlsmpm.MPM_FLAG_NO_LDS
+
+# Execution forms that must all give the same bits:
+#   direct  : per-particle global atomics, particles in reset order
+#   tiled   : LDS windows, particles in reset order
+#   sorted8 : LDS windows, counting sort by cell every 8 steps (the default)
+#   sorted1 : sort every step, 4 slices per lane (256-particle chunks)
+#   sorted32: sort every 3 steps, 32 slices per lane (2048-particle chunks)
+#   spill   : sort every 8 steps with a 64-cell window budget, so most chunks
+#             take the global-atomic fallback
+MODES = {
+    "direct": dict(flags=KEEP | NO_LDS, sort_period=0),
+    "tiled": dict(flags=KEEP, sort_period=0),
+    "sorted8": dict(flags=KEEP, sort_period=8),
+    "sorted1": dict(flags=KEEP, sort_period=1, tile_slices=4),
+    "sorted32": dict(flags=KEEP, sort_period=3, tile_slices=32),
+    "spill": dict(flags=KEEP, sort_period=8, tile_wmax=64),
+}
 
 
-def make_pair(grid, pos, vel=None, D=0, flags=KEEP, sort_period=0, seed=1):
+def make_pair(grid, pos, vel=None, D=0, flags=KEEP, sort_period=8, seed=1, **kw):
     cfg = mlsmpm.default_config(*grid, len(pos), diffuse_capacity=D, flags=flags,
-                                sort_period=sort_period, seed=seed)
+                                sort_period=sort_period, seed=seed, **kw)
     s = mlsmpm.Scene(cfg)
     s.reset(pos, vel)
     o = oracle.OracleScene(grid=grid, diffuse_max=D, seed=seed)
@@ -49,29 +67,32 @@ def check_equal(s, o, where, ints=True, diffuse=False):
             np.testing.assert_array_equal(rec[f], o.pool[f], err_msg=f"diffuse {f} {where}")
 
 
-def test_small_dam_break_every_step():
+@pytest.mark.parametrize("mode", list(MODES))
+def test_small_dam_break_every_step(mode):
     grid = (32, 32, 32)
     pos = scenes.lattice_block((2, 2, 2), (14, 20, 14), 0.1, np.random.default_rng(1))
-    s, o = make_pair(grid, pos)
+    s, o = make_pair(grid, pos, **MODES[mode])
     for step in range(12):
         s.step(1)
         o.step(1)
         check_equal(s, o, f"step {step}")
 
 
-def test_multi_step_call_matches():
+@pytest.mark.parametrize("mode", ["direct", "sorted8", "sorted32"])
+def test_multi_step_call_matches(mode):
     grid = (32, 24, 28)
     pos = scenes.lattice_block((3, 2, 2.5), (20, 15, 17), 0.1, np.random.default_rng(2))
-    s, o = make_pair(grid, pos)
+    s, o = make_pair(grid, pos, **MODES[mode])
     s.step(25)
     o.step(25)
     check_equal(s, o, "after 25 steps")
 
 
-def test_dam_break_64_sample():
+@pytest.mark.parametrize("mode", ["direct", "sorted8"])
+def test_dam_break_64_sample(mode):
     """64^3, block [2,30]x[2,40]x[2,30] (238,336 particles), first 3 steps."""
     grid, pos, _ = scenes.dam_break_small(seed=0)
-    s, o = make_pair(grid, pos)
+    s, o = make_pair(grid, pos, **MODES[mode])
     for step in range(3):
         s.step(1)
         o.step(1, threads=8)
@@ -95,34 +116,37 @@ def test_fused_clear_equals_separate_clear():
         np.testing.assert_array_equal(ga["cell_v"], gb["cell_v"])
 
 
-def test_diffuse_every_step():
+@pytest.mark.parametrize("mode", ["direct", "sorted8", "sorted1"])
+def test_diffuse_every_step(mode):
     grid = (32, 32, 32)
     pos = scenes.lattice_block((2, 2, 2), (14, 22, 14), 0.1, np.random.default_rng(4))
-    s, o = make_pair(grid, pos, D=20000)
+    s, o = make_pair(grid, pos, D=20000, **MODES[mode])
     for step in range(15):
         s.step(1)
         o.step(1)
         check_equal(s, o, f"step {step}", diffuse=True)
 
 
-def test_diffuse_pool_saturation():
+@pytest.mark.parametrize("mode", ["direct", "sorted8"])
+def test_diffuse_pool_saturation(mode):
     """A pool far smaller than the spawn rate: lowest particle index wins."""
     grid = (32, 32, 32)
     pos = scenes.lattice_block((2, 2, 2), (14, 22, 14), 0.1, np.random.default_rng(5))
-    s, o = make_pair(grid, pos, D=300)
+    s, o = make_pair(grid, pos, D=300, **MODES[mode])
     for step in range(10):
         s.step(1)
         o.step(1)
         check_equal(s, o, f"step {step}", diffuse=True)
 
 
-def test_non_unit_mass():
+@pytest.mark.parametrize("mode", ["direct", "sorted8"])
+def test_non_unit_mass(mode):
     grid = (28, 28, 28)
     rng = np.random.default_rng(6)
     pos = scenes.lattice_block((3, 3, 3), (12, 16, 12), 0.1, rng)
     mass = rng.uniform(0.5, 2.0, len(pos)).astype(np.float32)
     vel = rng.uniform(-1, 1, (len(pos), 3)).astype(np.float32)
-    cfg = mlsmpm.default_config(*grid, len(pos), flags=KEEP, sort_period=0)
+    cfg = mlsmpm.default_config(*grid, len(pos), **MODES[mode])
     with mlsmpm.Scene(cfg) as s:
         s.reset(pos, vel, mass=mass)
         o = oracle.OracleScene(grid=grid)
@@ -131,6 +155,23 @@ def test_non_unit_mass():
             s.step(1)
             o.step(1)
             check_equal(s, o, f"step {step}")
+
+
+@pytest.mark.parametrize("mode", ["direct", "sorted8", "spill"])
+def test_random_order_scattered(mode):
+    """Particles in random order with random velocities: before the first
+    sort every chunk spans the whole block and takes the fallback path."""
+    grid = (40, 36, 32)
+    rng = np.random.default_rng(8)
+    pos = scenes.lattice_block((3, 3, 3), (30, 20, 25), 0.1, rng)
+    perm = rng.permutation(len(pos))
+    pos = pos[perm]
+    vel = rng.uniform(-3, 3, pos.shape).astype(np.float32)
+    s, o = make_pair(grid, pos, vel, **MODES[mode])
+    for step in range(10):
+        s.step(1)
+        o.step(1)
+        check_equal(s, o, f"step {step}")
 
 
 def test_reset_rejects_out_of_grid():
