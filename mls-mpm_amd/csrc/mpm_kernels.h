@@ -31,6 +31,7 @@ struct SortBufs {
   uint32_t *offsets;    // per key, exclusive scan of counts
   uint32_t *scratch;    // scan scratch
   uint32_t *tile_rank;  // 4x4x4 grid tile -> position in Morton order
+  uint32_t *src_of;     // per new slot: old slot (G2P re-layout)
   uint32_t nkeys;       // 64 * number of tiles
 };
 
@@ -41,11 +42,13 @@ void launch_update_dense(const Params &P, const GState &g, const Walls &W, bool 
                          hipStream_t st);
 void launch_g2p_global(const Params &P, const PState &S, const GState &g, const Walls &W,
                        hipStream_t st);
+void launch_g2p_relayout(const Params &P, const PState &S, const PState &D, const GState &g,
+                         const Walls &W, const uint32_t *src_of, hipStream_t st);
 void launch_diffuse(const Params &P, const PState &S, const GState &g, const Walls &W,
                     const DiffuseBufs &D, hipStream_t st);
 uint32_t scan_scratch_words(uint32_t n);
 void launch_scan(const uint32_t *in, uint32_t n, bool popc, uint32_t *out, uint32_t *scratch,
-                 uint32_t *total, hipStream_t st);
+                 uint32_t *total, hipStream_t st, uint32_t *clear = nullptr);
 void launch_export_particles(const PState &S, uint32_t n, float4 *pos4, float4 *vel4, float *C9,
                              float *dens, hipStream_t st);
 void launch_export_grid(const GState &g, uint32_t G, int32_t *m, int32_t *mv3, float *mf,

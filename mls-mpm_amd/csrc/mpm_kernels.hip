@@ -104,11 +104,37 @@ __global__ __launch_bounds__(256) void k_g2p_global(Params P, PState S, GState g
   particle_weights(px, py, pz, pw);
   const uint32_t gx = P.gx, slice = P.gx * P.gy;
   uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
-  float3 np;
-  g2p_particle(P, W, S, p, px, py, pz, pw, [&](int k, int j, int i) {
+  G2POut o;
+  g2p_particle(P, W, px, py, pz, pw, [&](int k, int j, int i) {
     float4 v = g.v[base + k * slice + j * gx + i];
     return make_float3(v.x, v.y, v.z);
-  }, np);
+  }, o);
+  store_g2p(S, p, o);
+}
+
+// G2P that also applies the new particle order: destination slot d takes the
+// particle at source slot src_of[d], whose cell key and rank the P2G mass
+// pass recorded.  Reads are gathered from the old order, writes are
+// coalesced in the new one; mass, id and density move along.
+__global__ __launch_bounds__(256) void k_g2p_relayout(Params P, PState S, PState D, GState g,
+                                                      Walls W, const uint32_t *src_of) {
+  uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= P.n) return;
+  uint32_t p = src_of[d];
+  float px = S.at(PX, p), py = S.at(PY, p), pz = S.at(PZ, p);
+  PW pw;
+  particle_weights(px, py, pz, pw);
+  const uint32_t gx = P.gx, slice = P.gx * P.gy;
+  uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
+  G2POut o;
+  g2p_particle(P, W, px, py, pz, pw, [&](int k, int j, int i) {
+    float4 v = g.v[base + k * slice + j * gx + i];
+    return make_float3(v.x, v.y, v.z);
+  }, o);
+  store_g2p(D, d, o);
+  D.at(PM, d) = S.at(PM, p);
+  D.id[d] = S.id[p];
+  D.density[d] = S.density[p];
 }
 
 // ---------------------------------------------------------------------------
@@ -421,10 +447,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(uint32_t *partia
   if (threadIdx.x == 0 && total) *total = carry;
 }
 
+// `clear` (the input array itself, or null) is zeroed behind the scan, which
+// leaves a histogram ready for its next use without a separate fill.
 template <bool POPC>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in, uint32_t n,
                                                            const uint32_t *partials,
-                                                           uint32_t *out) {
+                                                           uint32_t *out, uint32_t *clear) {
   uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
@@ -438,7 +466,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in, 
   uint32_t run = partials[blockIdx.x] + excl;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    if (base + i < n) out[base + i] = run;
+    if (base + i < n) {
+      out[base + i] = run;
+      if (clear) clear[base + i] = 0u;
+    }
     run += v[i];
   }
 }
@@ -524,18 +555,23 @@ void launch_g2p_global(const Params &P, const PState &S, const GState &g, const 
   if (!P.n) return;
   k_g2p_global<<<blocks(P.n, 256), 256, 0, st>>>(P, S, g, W);
 }
+void launch_g2p_relayout(const Params &P, const PState &S, const PState &D, const GState &g,
+                         const Walls &W, const uint32_t *src_of, hipStream_t st) {
+  if (!P.n) return;
+  k_g2p_relayout<<<blocks(P.n, 256), 256, 0, st>>>(P, S, D, g, W, src_of);
+}
 
 uint32_t scan_scratch_words(uint32_t n) { return blocks(n, kScanTile) + 1; }
 
 void launch_scan(const uint32_t *in, uint32_t n, bool popc, uint32_t *out, uint32_t *scratch,
-                 uint32_t *total, hipStream_t st) {
+                 uint32_t *total, hipStream_t st, uint32_t *clear) {
   uint32_t nb = blocks(n, kScanTile);
   if (nb == 0) nb = 1;
   if (popc) k_scan_reduce<true><<<nb, kScanThreads, 0, st>>>(in, n, scratch);
   else k_scan_reduce<false><<<nb, kScanThreads, 0, st>>>(in, n, scratch);
   k_scan_partials<<<1, kScanThreads, 0, st>>>(scratch, nb, total);
-  if (popc) k_scan_down<true><<<nb, kScanThreads, 0, st>>>(in, n, scratch, out);
-  else k_scan_down<false><<<nb, kScanThreads, 0, st>>>(in, n, scratch, out);
+  if (popc) k_scan_down<true><<<nb, kScanThreads, 0, st>>>(in, n, scratch, out, clear);
+  else k_scan_down<false><<<nb, kScanThreads, 0, st>>>(in, n, scratch, out, clear);
 }
 
 void launch_diffuse(const Params &P, const PState &S, const GState &g, const Walls &W,

@@ -141,10 +141,14 @@ __device__ __forceinline__ float4 update_cell(const Params &P, const Walls &W, i
 
 // G2P of one particle (WGSL_G2P, shaders.js:589-704) given the 27 gathered
 // cell velocities through `vel_at(k,j,i)`.  Writes pos, vel, C.
+struct G2POut {
+  float C[9];
+  float3 pos, vel;
+};
+
 template <class VelAt>
-__device__ __forceinline__ void g2p_particle(const Params &P, const Walls &W, const PState &S,
-                                             uint32_t p, float px, float py, float pz,
-                                             const PW &pw, VelAt vel_at, float3 &np_out) {
+__device__ __forceinline__ void g2p_particle(const Params &P, const Walls &W, float px, float py,
+                                             float pz, const PW &pw, VelAt vel_at, G2POut &o) {
   float gvx = 0.0f, gvy = 0.0f, gvz = 0.0f;
   float B00 = 0.0f, B01 = 0.0f, B02 = 0.0f;
   float B10 = 0.0f, B11 = 0.0f, B12 = 0.0f;
@@ -175,9 +179,9 @@ __device__ __forceinline__ void g2p_particle(const Params &P, const Walls &W, co
       }
     }
   }
-  S.at(C00, p) = B00 * 4.0f; S.at(C01, p) = B01 * 4.0f; S.at(C02, p) = B02 * 4.0f;
-  S.at(C10, p) = B10 * 4.0f; S.at(C11, p) = B11 * 4.0f; S.at(C12, p) = B12 * 4.0f;
-  S.at(C20, p) = B20 * 4.0f; S.at(C21, p) = B21 * 4.0f; S.at(C22, p) = B22 * 4.0f;
+  o.C[0] = B00 * 4.0f; o.C[1] = B01 * 4.0f; o.C[2] = B02 * 4.0f;
+  o.C[3] = B10 * 4.0f; o.C[4] = B11 * 4.0f; o.C[5] = B12 * 4.0f;
+  o.C[6] = B20 * 4.0f; o.C[7] = B21 * 4.0f; o.C[8] = B22 * 4.0f;
 
   const float dt = P.dt, lk = P.lookahead_k;
   F3 np = f3(px + dt * gvx, py + dt * gvy, pz + dt * gvz);
@@ -226,9 +230,20 @@ __device__ __forceinline__ void g2p_particle(const Params &P, const Walls &W, co
   if (np.y > P.hard_max_y) { np.y = P.hard_max_y; nv.y = 0.0f; }
   if (np.z < P.hard_min) { np.z = P.hard_min; nv.z = 0.0f; }
   if (np.z > P.hard_max_z) { np.z = P.hard_max_z; nv.z = 0.0f; }
-  S.at(PX, p) = np.x; S.at(PY, p) = np.y; S.at(PZ, p) = np.z;
-  S.at(VX, p) = nv.x; S.at(VY, p) = nv.y; S.at(VZ, p) = nv.z;
-  np_out = make_float3(np.x, np.y, np.z);
+  o.pos = make_float3(np.x, np.y, np.z);
+  o.vel = make_float3(nv.x, nv.y, nv.z);
+}
+
+// Store a G2P result into slot d of a particle state.
+__device__ __forceinline__ void store_g2p(const PState &S, uint32_t d, const G2POut &o) {
+#pragma unroll
+  for (int k = 0; k < 9; k++) S.at(C00 + k, d) = o.C[k];
+  S.at(PX, d) = o.pos.x;
+  S.at(PY, d) = o.pos.y;
+  S.at(PZ, d) = o.pos.z;
+  S.at(VX, d) = o.vel.x;
+  S.at(VY, d) = o.vel.y;
+  S.at(VZ, d) = o.vel.z;
 }
 
 }  // namespace mpm

@@ -17,7 +17,7 @@ struct mpm_scene {
   int cur = 0;           // which particle buffer is current (sorting ping-pongs)
   float *planes[2] = {nullptr, nullptr};
   uint32_t *ids[2] = {nullptr, nullptr};
-  float *density = nullptr;
+  float *dens[2] = {nullptr, nullptr};  // particle_density, per layout buffer
   int unit_mass = 1;
   mpm::GState g{};
   mpm::DiffuseBufs d{};

@@ -141,7 +141,7 @@ static PState pstate(const mpm_scene *s, int which) {
   PState S;
   S.pl = s->planes[which];
   S.id = s->ids[which];
-  S.density = s->density;
+  S.density = s->dens[which];
   S.cap = s->cfg.capacity ? s->cfg.capacity : 1;
   return S;
 }
@@ -204,7 +204,8 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     DALLOC(s->planes[b], cap * NPLANES * 4);
     DALLOC(s->ids[b], cap * 4);
   }
-  DALLOC(s->density, cap * 4);
+  DALLOC(s->dens[0], cap * 4);
+  DALLOC(s->dens[1], cap * 4);
   // tiled P2G and the sort
   s->tc.slices = c.tile_slices ? c.tile_slices : 16;
   s->tc.wmax = c.tile_wmax ? c.tile_wmax : 3072;
@@ -221,6 +222,8 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     DALLOC(s->sort.offsets, (size_t)s->sort.nkeys * 4);
     DALLOC(s->sort.scratch, (size_t)scan_scratch_words(s->sort.nkeys) * 4);
     DALLOC(s->sort.tile_rank, (size_t)ntiles * 4);
+    DALLOC(s->sort.src_of, cap * 4);
+    HIPCHK(hipMemsetAsync(s->sort.counts, 0, (size_t)s->sort.nkeys * 4, s->stream));
     std::vector<uint32_t> rank = morton_tile_rank(s->tdx, s->tdy, s->tdz);
     HIPCHK(hipMemcpy(s->sort.tile_rank, rank.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice));
   }
@@ -416,19 +419,31 @@ int scene_step(mpm_scene *s, uint32_t nsteps) {
     };
     int rc = mark(0);
     if (rc) return rc;
-    if (s->cfg.sort_period && s->n &&
-        (!s->sorted_once || s->since_sort >= s->cfg.sort_period)) {
-      PState dst = pstate(s, s->cur ^ 1);
-      launch_sort(P, S, dst, s->sort, s->tdx, s->tdy, s->tc.slices, st);
-      s->cur ^= 1;
-      S = dst;
-      s->since_sort = 0;
-      s->sorted_once = true;
+    const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS);
+    const uint32_t period = s->cfg.sort_period;
+    // Layout policy: the first step after a reset sorts the caller-order
+    // particles with a full sort; afterwards every `period` steps the P2G mass
+    // pass records cell keys and G2P writes the new state in sorted order.
+    // The direct path has no key recording and sorts with the full sort.
+    bool relayout = false;
+    if (period && s->n) {
+      bool due = !s->sorted_once || s->since_sort >= period;
+      if (due && (!tiled || !s->sorted_once)) {
+        PState dst = pstate(s, s->cur ^ 1);
+        launch_sort(P, S, dst, s->sort, s->tdx, s->tdy, s->tc.slices, st);
+        s->cur ^= 1;
+        S = dst;
+        s->since_sort = 0;
+        s->sorted_once = true;
+      } else if (due) {
+        relayout = true;
+      }
     }
     if ((rc = mark(1))) return rc;
-    const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS);
     if (keep) launch_clear_grid(s->g, s->G, st);
-    if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, s->t.boxes, st);
+    KeyOut ko{};
+    if (relayout) ko = key_out(s->sort, s->tdx, s->tdy);
+    if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, s->t.boxes, ko, st);
     else launch_p2g_mass_global(P, S, s->g, st);
     if ((rc = mark(2))) return rc;
     if (tiled) launch_p2g_mom_tiled(P, S, s->g, s->tc, s->t.boxes, st);
@@ -436,7 +451,16 @@ int scene_step(mpm_scene *s, uint32_t nsteps) {
     if ((rc = mark(3))) return rc;
     launch_update_dense(P, s->g, s->walls, !keep, st);
     if ((rc = mark(4))) return rc;
-    launch_g2p_global(P, S, s->g, s->walls, st);
+    if (relayout) {
+      launch_relayout_prep(P, s->sort, s->tc.slices, st);
+      PState dst = pstate(s, s->cur ^ 1);
+      launch_g2p_relayout(P, S, dst, s->g, s->walls, s->sort.src_of, st);
+      s->cur ^= 1;
+      S = dst;
+      s->since_sort = 0;
+    } else {
+      launch_g2p_global(P, S, s->g, s->walls, st);
+    }
     if ((rc = mark(5))) return rc;
     launch_diffuse(P, S, s->g, s->walls, s->d, st);
     if ((rc = mark(6))) return rc;

@@ -56,11 +56,31 @@ __device__ __forceinline__ uint32_t window_gidx(const Box &bx, const Params &P, 
          (uint32_t)(bx.x0 + (int)cx);
 }
 
+// Sort key of the cell holding a particle: Morton rank of its 4x4x4 tile,
+// then x-fastest position inside the tile.
+__device__ __forceinline__ uint32_t cell_key(const Params &P, const KeyOut &ko, float x, float y,
+                                             float z) {
+  int cx = min(max((int)floorf(x), 0), (int)P.gx - 1);
+  int cy = min(max((int)floorf(y), 0), (int)P.gy - 1);
+  int cz = min(max((int)floorf(z), 0), (int)P.gz - 1);
+  uint32_t tile = ((uint32_t)(cz >> 2) * ko.tdy + (uint32_t)(cy >> 2)) * ko.tdx + (uint32_t)(cx >> 2);
+  return ko.tile_rank[tile] * 64u + (uint32_t)(((cz & 3) << 4) | ((cy & 3) << 2) | (cx & 3));
+}
+__device__ __forceinline__ void record_key(const Params &P, const KeyOut &ko, uint32_t s, float x,
+                                           float y, float z) {
+  uint32_t key = cell_key(P, ko, x, y, z);
+  ko.keys[s] = key;
+  ko.rank[s] = atomicAdd(&ko.counts[key], 1u);
+}
+
 // ---------------------------------------------------------------------------
-// P2G mass, tiled.  Also writes the chunk's window for the momentum pass.
+// P2G mass, tiled.  Also writes the chunk's window for the momentum pass and,
+// when ko.counts is set, the cell key and rank of every particle for the
+// re-layout done by G2P.
 template <int SL, bool UNIT>
 __global__ __launch_bounds__(kTB) void k_p2g_mass_tiled(Params P, PState S, GState g,
-                                                        int32_t *boxes, uint32_t wmax) {
+                                                        int32_t *boxes, uint32_t wmax,
+                                                        KeyOut ko) {
   constexpr int PER = SL / 4;  // particles per thread
   constexpr int CH = 64 * SL;
   extern __shared__ int32_t lds[];
@@ -79,6 +99,7 @@ __global__ __launch_bounds__(kTB) void k_p2g_mass_tiled(Params P, PState S, GSta
       py[r] = S.at(PY, s);
       pz[r] = S.at(PZ, s);
       int bx = (int)floorf(px[r]) - 1, by = (int)floorf(py[r]) - 1, bz = (int)floorf(pz[r]) - 1;
+      if (ko.counts) record_key(P, ko, s, px[r], py[r], pz[r]);
       mn[0] = min(mn[0], bx); mx[0] = max(mx[0], bx);
       mn[1] = min(mn[1], by); mx[1] = max(mx[1], by);
       mn[2] = min(mn[2], bz); mx[2] = max(mx[2], bz);
@@ -249,19 +270,29 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *a, uint32_t n, uint3
   if (i < n) a[i] = v;
 }
 
-__global__ __launch_bounds__(256) void k_sort_keys(Params P, PState S, const uint32_t *tile_rank,
-                                                   uint32_t tdx, uint32_t tdy, uint32_t *counts,
-                                                   uint32_t *keys, uint32_t *rank) {
+__global__ __launch_bounds__(256) void k_sort_keys(Params P, PState S, KeyOut ko) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
-  int cx = (int)floorf(S.at(PX, s)), cy = (int)floorf(S.at(PY, s)), cz = (int)floorf(S.at(PZ, s));
-  cx = min(max(cx, 0), (int)P.gx - 1);
-  cy = min(max(cy, 0), (int)P.gy - 1);
-  cz = min(max(cz, 0), (int)P.gz - 1);
-  uint32_t tile = ((uint32_t)(cz >> 2) * tdy + (uint32_t)(cy >> 2)) * tdx + (uint32_t)(cx >> 2);
-  uint32_t key = tile_rank[tile] * 64u + (uint32_t)(((cz & 3) << 4) | ((cy & 3) << 2) | (cx & 3));
-  keys[s] = key;
-  rank[s] = atomicAdd(&counts[key], 1u);
+  record_key(P, ko, s, S.at(PX, s), S.at(PY, s), S.at(PZ, s));
+}
+
+// Slot of sorted rank q: inside a full chunk of 64*sl particles, rank
+// lane*sl + slice goes to slice*64 + lane; the last partial chunk keeps order.
+__device__ __forceinline__ uint32_t slot_of_rank(uint32_t q, uint32_t n, uint32_t sl) {
+  const uint32_t ch = 64u * sl;
+  uint32_t full = (n / ch) * ch;
+  if (q >= full) return q;
+  uint32_t r = q % ch;
+  return (q - r) + (r % sl) * 64u + r / sl;
+}
+
+// Inverse permutation for the G2P re-layout: src_of[new slot] = old slot.
+__global__ __launch_bounds__(256) void k_build_src(uint32_t n, uint32_t sl, const uint32_t *keys,
+                                                   const uint32_t *rank, const uint32_t *offsets,
+                                                   uint32_t *src_of) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  src_of[slot_of_rank(offsets[keys[s]] + rank[s], n, sl)] = s;
 }
 
 __global__ __launch_bounds__(256) void k_sort_scatter(uint32_t n, uint32_t sl, PState src,
@@ -270,17 +301,11 @@ __global__ __launch_bounds__(256) void k_sort_scatter(uint32_t n, uint32_t sl, P
                                                       const uint32_t *offsets) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const uint32_t ch = 64u * sl;
-  uint32_t q = offsets[keys[s]] + rank[s];
-  uint32_t full = (n / ch) * ch;
-  uint32_t d = q;
-  if (q < full) {
-    uint32_t r = q % ch;
-    d = (q - r) + (r % sl) * 64u + r / sl;
-  }
+  uint32_t d = slot_of_rank(offsets[keys[s]] + rank[s], n, sl);
 #pragma unroll
   for (int k = 0; k < NPLANES; k++) dst.at(k, d) = src.at(k, s);
   dst.id[d] = src.id[s];
+  dst.density[d] = src.density[s];
 }
 
 // ---------------------------------------------------------------------------
@@ -289,11 +314,11 @@ static inline uint32_t nblk(uint64_t n, uint32_t t) { return (uint32_t)((n + t -
 
 template <int SL>
 static void mass_tiled(const Params &P, const PState &S, const GState &g, int32_t *boxes,
-                       uint32_t wmax, hipStream_t st) {
+                       uint32_t wmax, const KeyOut &ko, hipStream_t st) {
   uint32_t nb = nblk(P.n, 64u * SL);
   size_t shm = (size_t)wmax * 4;
-  if (P.unit_mass) k_p2g_mass_tiled<SL, true><<<nb, kTB, shm, st>>>(P, S, g, boxes, wmax);
-  else k_p2g_mass_tiled<SL, false><<<nb, kTB, shm, st>>>(P, S, g, boxes, wmax);
+  if (P.unit_mass) k_p2g_mass_tiled<SL, true><<<nb, kTB, shm, st>>>(P, S, g, boxes, wmax, ko);
+  else k_p2g_mass_tiled<SL, false><<<nb, kTB, shm, st>>>(P, S, g, boxes, wmax, ko);
 }
 template <int SL>
 static void mom_tiled(const Params &P, const PState &S, const GState &g, const int32_t *boxes,
@@ -307,13 +332,13 @@ static void mom_tiled(const Params &P, const PState &S, const GState &g, const i
 bool tiled_slices_supported(uint32_t sl) { return sl == 4 || sl == 8 || sl == 16 || sl == 32; }
 
 void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
-                           int32_t *boxes, hipStream_t st) {
+                           int32_t *boxes, const KeyOut &ko, hipStream_t st) {
   if (!P.n) return;
   switch (tc.slices) {
-    case 4: mass_tiled<4>(P, S, g, boxes, tc.wmax, st); break;
-    case 8: mass_tiled<8>(P, S, g, boxes, tc.wmax, st); break;
-    case 16: mass_tiled<16>(P, S, g, boxes, tc.wmax, st); break;
-    default: mass_tiled<32>(P, S, g, boxes, tc.wmax, st); break;
+    case 4: mass_tiled<4>(P, S, g, boxes, tc.wmax, ko, st); break;
+    case 8: mass_tiled<8>(P, S, g, boxes, tc.wmax, ko, st); break;
+    case 16: mass_tiled<16>(P, S, g, boxes, tc.wmax, ko, st); break;
+    default: mass_tiled<32>(P, S, g, boxes, tc.wmax, ko, st); break;
   }
 }
 
@@ -328,15 +353,36 @@ void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, con
   }
 }
 
+KeyOut key_out(const SortBufs &sb, uint32_t tdx, uint32_t tdy) {
+  KeyOut ko;
+  ko.counts = sb.counts;
+  ko.keys = sb.keys;
+  ko.rank = sb.rank;
+  ko.tile_rank = sb.tile_rank;
+  ko.tdx = tdx;
+  ko.tdy = tdy;
+  return ko;
+}
+
+// Full sort (data moved by a scatter pass); used for the first step after a
+// reset, when the particles are still in caller order.  The histogram array
+// is left zeroed.
 void launch_sort(const Params &P, const PState &src, const PState &dst, const SortBufs &sb,
                  uint32_t tdx, uint32_t tdy, uint32_t slices, hipStream_t st) {
   if (!P.n) return;
-  k_fill_u32<<<nblk(sb.nkeys, 256), 256, 0, st>>>(sb.counts, sb.nkeys, 0u);
-  k_sort_keys<<<nblk(P.n, 256), 256, 0, st>>>(P, src, sb.tile_rank, tdx, tdy, sb.counts, sb.keys,
-                                              sb.rank);
-  launch_scan(sb.counts, sb.nkeys, false, sb.offsets, sb.scratch, nullptr, st);
+  k_sort_keys<<<nblk(P.n, 256), 256, 0, st>>>(P, src, key_out(sb, tdx, tdy));
+  launch_scan(sb.counts, sb.nkeys, false, sb.offsets, sb.scratch, nullptr, st, sb.counts);
   k_sort_scatter<<<nblk(P.n, 256), 256, 0, st>>>(P.n, slices, src, dst, sb.keys, sb.rank,
                                                  sb.offsets);
+}
+
+// After a P2G mass pass that recorded keys: offsets and the inverse
+// permutation for k_g2p_relayout.  Leaves the histogram zeroed.
+void launch_relayout_prep(const Params &P, const SortBufs &sb, uint32_t slices, hipStream_t st) {
+  if (!P.n) return;
+  launch_scan(sb.counts, sb.nkeys, false, sb.offsets, sb.scratch, nullptr, st, sb.counts);
+  k_build_src<<<nblk(P.n, 256), 256, 0, st>>>(P.n, slices, sb.keys, sb.rank, sb.offsets,
+                                              sb.src_of);
 }
 
 }  // namespace mpm

@@ -1,0 +1,417 @@
+"""Pins of the CPU restatement (oracle/mpm_oracle.c) by closed forms and
+invariants.
+
+The reference ships no tests, fixtures or golden vectors for this path and
+its WGSL cannot run here, so each oracle function is pinned by what the
+mathematics fixes, chosen so that a plausible restatement mistake (a dropped
+term, a wrong sign or index, a transposed operand) fails a check.  The pins
+compute their expectations independently (float64, textbook identities);
+they do not retype the oracle's f32 expressions.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+F32 = np.float32
+
+
+def single(grid=(16, 16, 16), **pk):
+    s = O.OracleScene(grid=grid)
+    for k, v in pk.items():
+        setattr(s.params, k, v)
+    return s
+
+
+# ---- encode / decode (shaders.js:15-25) -----------------------------------
+def test_encode_truncates_toward_zero_and_saturates():
+    assert O.encode(1.5) == 150000
+    assert O.encode(2.7e-5) == 2          # 2.7 -> 2: truncation, not rounding
+    assert O.encode(-2.7e-5) == -2        # toward zero for negatives
+    assert O.encode(0.9e-5) == 0
+    assert O.encode(19999.0) == 1999900032   # f32 product: spacing 128 near 2^31
+    assert O.encode(20000.0) == 2000000000
+    assert O.encode(1e9) == 2000000000     # clamp at +20000 before scaling
+    assert O.encode(-1e30) == -2000000000
+    assert O.encode(float("inf")) == 2000000000
+
+
+def test_encode_properties():
+    rng = np.random.default_rng(0)
+    v = np.concatenate([rng.uniform(-30000, 30000, 2000), rng.normal(0, 1, 2000),
+                        rng.normal(0, 1e-3, 2000)]).astype(F32)
+    e = np.array([O.encode(float(x)) for x in v], dtype=np.int64)
+    # odd symmetry and monotonicity
+    e_neg = np.array([O.encode(float(-x)) for x in v], dtype=np.int64)
+    assert np.array_equal(e_neg, -e)
+    order = np.argsort(v, kind="stable")
+    assert np.all(np.diff(e[order]) >= 0)
+    # |encode(v) - 1e5 v| < 1 (+ f32 rounding of the product) inside the range
+    inside = np.abs(v) <= 20000
+    ref = np.clip(v.astype(np.float64), -20000, 20000) * 1e5
+    assert np.all(np.abs(e[inside] - ref[inside]) <= 1 + np.abs(ref[inside]) * 2 ** -23)
+
+
+def test_decode_inverts_encode_to_fixed_point_resolution():
+    for x in [0.0, 1.0, -3.25, 12.345678, -0.00042, 19999.5]:
+        d = O.decode(O.encode(x))
+        assert abs(d - x) <= 1e-5 + abs(x) * 2 ** -22
+
+
+# ---- weights (shaders.js:53-59, 216-240) -----------------------------------
+@pytest.mark.parametrize("x", [5.5, 5.0, 5.999, 7.25, 3.75, 10.5001])
+def test_weights_partition_and_moments(x):
+    s = single()
+    s.reset(np.array([[x, 6.5, 7.5]]))
+    s.k_weights()
+    bw = s.bw[0]
+    assert bw["b"][0] == int(np.floor(x)) - 1
+    f = float(bw["f"][0])
+    assert abs(f - (x - np.floor(x) - 0.5)) < 1e-6
+    w = bw["w"][0].astype(np.float64)
+    d = np.array([-1.0, 0.0, 1.0]) - f                 # node - particle offsets
+    assert abs(w.sum() - 1.0) < 1e-6                    # partition of unity
+    assert abs((w * d).sum()) < 1e-6                    # first moment
+    assert abs((w * d * d).sum() - 0.25) < 1e-6         # second moment = 1/4
+    assert np.all(w >= 0)
+
+
+def test_weights_at_cell_centre():
+    s = single()
+    s.reset(np.array([[5.5, 6.5, 7.5]]))
+    s.k_weights()
+    assert np.array_equal(s.bw[0]["w"][1], np.array([0.125, 0.75, 0.125], F32))
+    assert list(s.bw[0]["b"]) == [4, 5, 6]
+
+
+# ---- P2G mass (shaders.js:242-274) -----------------------------------------
+def test_p2g_mass_single_particle_layout_and_sum():
+    g = (12, 14, 16)
+    s = single(g)
+    p = np.array([[4.3, 6.8, 9.1]])
+    s.reset(p)
+    s.k_clear(); s.k_weights(); s.k_p2g_mass()
+    gx, gy, gz = g
+    m = s.cell_mass.reshape(gz, gy, gx)         # x fastest: idx = (cz*gy + cy)*gx + cx
+    nz = np.argwhere(m != 0)
+    base = np.floor(p[0]).astype(int) - 1
+    assert nz[:, 2].min() == base[0] and nz[:, 2].max() == base[0] + 2
+    assert nz[:, 1].min() == base[1] and nz[:, 1].max() == base[1] + 2
+    assert nz[:, 0].min() == base[2] and nz[:, 0].max() == base[2] + 2
+    # each cell holds trunc(1e5 * wx*wy*wz) up to one unit of f32 rounding
+    f = p[0] - np.floor(p[0]) - 0.5
+    w1 = [np.array([0.5 * (0.5 - t) ** 2, 0.75 - t * t, 0.5 * (0.5 + t) ** 2]) for t in f]
+    exp = np.einsum("k,j,i->kji", w1[2], w1[1], w1[0]) * 1e5
+    got = m[base[2]:base[2] + 3, base[1]:base[1] + 3, base[0]:base[0] + 3]
+    assert np.all(np.abs(got - np.trunc(exp)) <= 1)
+    assert 1e5 - 27 <= m.sum() <= 1e5
+
+
+def test_p2g_mass_is_a_function_of_the_particle_multiset():
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(2.2, 9.8, (500, 3))
+    a, b = single((12, 12, 12)), single((12, 12, 12))
+    a.reset(pos)
+    b.reset(pos[rng.permutation(len(pos))])
+    for s in (a, b):
+        s.k_clear(); s.k_weights(); s.k_p2g_mass()
+    assert np.array_equal(a.cell_mass, b.cell_mass)
+
+
+# ---- P2G momentum (shaders.js:276-371) -------------------------------------
+def _mom_moments(s, p):
+    """Sum of cell momenta and their first moment about the particle."""
+    gx, gy, gz = s.grid
+    mv = s.cell_mv.reshape(gz, gy, gx, 3).astype(np.float64) / 1e5
+    zz, yy, xx = np.meshgrid(np.arange(gz) + 0.5, np.arange(gy) + 0.5, np.arange(gx) + 0.5,
+                             indexing="ij")
+    dx = np.stack([xx - p[0], yy - p[1], zz - p[2]], axis=-1)
+    total = mv.reshape(-1, 3).sum(0)
+    moment = mv.reshape(-1, 3).T @ dx.reshape(-1, 3)   # sum_c mv_c[a] * dx_c[b]
+    return total, moment
+
+
+def test_p2g_mom_apic_moment_pins_C_orientation():
+    """stiffness = viscosity = 0: sum_c mv_c dx_c^T = m C D with D = I/4,
+    so an asymmetric C must come back un-transposed; sum_c mv_c = m v."""
+    s = single(stiffness=0.0, viscosity=0.0)
+    p = np.array([7.3, 8.6, 6.2])
+    A = np.array([[0.3, -1.2, 0.5], [2.0, 0.1, -0.7], [-0.4, 0.9, 1.1]])
+    v = np.array([0.25, -0.5, 1.5])
+    s.reset(p[None], vel=v[None], C9=A.reshape(1, 9), mass=np.array([2.0], F32))
+    s.k_clear(); s.k_weights(); s.k_p2g_mass(); s.k_decode_mass(); s.k_p2g_mom()
+    total, moment = _mom_moments(s, p)
+    assert np.allclose(total, 2.0 * v, atol=27 * 3e-5)
+    assert np.allclose(moment, 2.0 * A / 4.0, atol=2e-4)
+
+
+def test_p2g_mom_pressure_force_sign_and_size():
+    """v = C = 0, viscosity 0, 60 coincident particles: density exceeds rest,
+    so p > 0 and sum_c f_c dx_c^T = coef * s / 4 = vol*dt*p*I (outward)."""
+    n = 60
+    s = single(viscosity=0.0)
+    p = np.array([7.5, 8.5, 6.5])
+    s.reset(np.repeat(p[None], n, 0))
+    s.k_clear(); s.k_weights(); s.k_p2g_mass(); s.k_decode_mass(); s.k_p2g_mom()
+    rho = float(s.density[0])
+    w1 = np.array([0.125, 0.75, 0.125])
+    assert abs(rho - n * (w1 ** 2).sum() ** 3) < 1e-3 * rho   # density = sum_c w_c m_c
+    par = s.params
+    pressure = par.stiffness * (rho / par.rest_density - 1.0)
+    assert pressure > 0
+    vol = 1.0 / max(rho, par.rest_density * 0.15)
+    total, moment = _mom_moments(s, p)
+    assert np.allclose(total, 0.0, atol=n * 27e-5)
+    expect = n * vol * par.dt * pressure * np.eye(3)
+    assert np.allclose(moment, expect, rtol=1e-3, atol=n * 1e-4)
+
+
+def test_eos_cohesion_gate():
+    """Pressure floor: 0 below 0.7 rest density, -0.1 above (shaders.js:12-13).
+    Probed through the momentum moment with coincident particles."""
+    def moment_for(n):
+        s = single(viscosity=0.0)
+        p = np.array([7.5, 8.5, 6.5])
+        s.reset(np.repeat(p[None], n, 0))
+        s.k_clear(); s.k_weights(); s.k_p2g_mass(); s.k_decode_mass(); s.k_p2g_mom()
+        return float(s.density[0]), _mom_moments(s, p)[1]
+    rho_lo, m_lo = moment_for(20)      # ~4.2 < 5.6 = 0.7*rest -> floor 0 -> no force
+    assert rho_lo < 5.6 and np.allclose(m_lo, 0.0, atol=1e-3)
+    rho_mid, m_mid = moment_for(30)    # ~6.3 in (5.6, 8): p = max(100*(rho/8-1), -0.1) = -0.1
+    assert 5.6 < rho_mid < 8.0
+    vol = 1.0 / rho_mid
+    # small forces: fixed-point truncation of 27 x 30 contributions costs ~2%
+    assert np.allclose(np.diag(m_mid), 30 * vol * 0.3 * -0.1, rtol=5e-2)
+
+
+# ---- grid update (shaders.js:435-513) --------------------------------------
+def test_update_grid_table():
+    g = (10, 10, 10)
+    s = single(g)
+    gx, gy, gz = g
+    idx = lambda x, y, z: (z * gy + y) * gx + x
+    cases = {
+        (5, 5, 5): (2.0, (1.0, 1.0, -0.5)),    # interior
+        (1, 5, 5): (2.0, (1.0, 1.0, -0.5)),    # x band: vx = 0
+        (5, 1, 5): (2.0, (1.0, -1.0, -0.5)),   # floor band, downward vy -> 0, friction
+        (5, 0, 5): (2.0, (1.0, 3.0, 1.0)),     # floor band, upward vy kept, friction
+        (5, 8, 5): (2.0, (0.0, 5.0, 0.0)),     # ceiling band, upward vy -> 0
+        (5, 5, 8): (2.0, (0.0, 0.0, 2.0)),     # z band: vz = 0
+        (4, 4, 4): (0.0, (3.0, 3.0, 3.0)),     # no mass -> zero velocity
+    }
+    for (x, y, z), (m, mv) in cases.items():
+        c = idx(x, y, z)
+        s.cell_mass[c] = int(m * 1e5)
+        s.cell_mv[3 * c:3 * c + 3] = [int(t * 1e5) for t in mv]
+    s.k_update()
+    v = s.cell_v3()
+    gdt = s.params.gravity * s.params.dt
+    def at(x, y, z):
+        return v[idx(x, y, z)].astype(np.float64)
+    assert np.allclose(at(5, 5, 5), [0.5, 0.5 + gdt, -0.25], atol=1e-6)
+    assert np.allclose(at(1, 5, 5), [0.0, 0.5 + gdt, -0.25], atol=1e-6)
+    assert np.allclose(at(5, 1, 5), [0.5 * 0.85, 0.0, -0.25 * 0.9], atol=1e-6)
+    assert np.allclose(at(5, 0, 5), [0.5 * 0.85, 1.5 + gdt, 0.5 * 0.9], atol=1e-6)
+    assert np.allclose(at(5, 8, 5), [0.0, 0.0, 0.0], atol=1e-6)
+    assert np.allclose(at(5, 5, 8), [0.0, gdt, 0.0], atol=1e-6)
+    assert np.allclose(at(4, 4, 4), [0.0, 0.0, 0.0])
+    assert s.cell_v[0] == 0.0
+
+
+# ---- G2P (shaders.js:579-705) ----------------------------------------------
+def _linear_field(s, A, b):
+    gx, gy, gz = s.grid
+    zz, yy, xx = np.meshgrid(np.arange(gz) + 0.5, np.arange(gy) + 0.5, np.arange(gx) + 0.5,
+                             indexing="ij")
+    X = np.stack([xx, yy, zz], -1).reshape(-1, 3)
+    s.cell_v[:] = (X @ A.T + b).reshape(-1).astype(F32)
+
+
+def test_g2p_reproduces_linear_field_exactly():
+    """Quadratic B-spline APIC: a linear grid field v = A x + b gives
+    particle velocity A x_p + b and C = A (4 * sum w v dx^T = A)."""
+    s = single((20, 24, 20))
+    A = np.array([[0.1, -0.2, 0.05], [0.3, 0.0, -0.15], [-0.05, 0.2, 0.1]])
+    b = np.array([0.5, 0.25, -0.3])
+    _linear_field(s, A, b)
+    p = np.array([[9.37, 14.61, 8.12]])
+    s.reset(p)
+    s.k_weights(); s.k_g2p()
+    vexp = A @ p[0] + b
+    assert np.allclose(s.vel4[0, :3], vexp, rtol=1e-5, atol=1e-5)
+    assert np.allclose(s.affine[0].reshape(3, 3), A, atol=2e-5)
+    assert np.allclose(s.pos4[0, :3], p[0] + s.params.dt * vexp, atol=1e-5)
+    assert s.pos4[0, 3] == 0 and s.vel4[0, 3] == 0
+
+
+def test_g2p_floor_spring_and_hard_clamp():
+    s = single((16, 16, 16))
+    s.cell_v[:] = 0
+    s.reset(np.array([[8.5, 3.0, 8.5], [8.5, 9.0, 8.5]]))
+    s.k_weights(); s.k_g2p()
+    # floor zone y < hard_min + 3 = 5: dist = 2, blend = 2/3, vy += 1 * 2/3 * 2
+    assert abs(s.vel4[0, 1] - 4.0 / 3.0) < 1e-6
+    assert s.vel4[1, 1] == 0.0
+    # outward velocity past hard_max is clamped and zeroed
+    s2 = single((16, 16, 16))
+    s2.cell_v[:] = np.tile(np.array([10.0, 0.0, 0.0], F32), 16 ** 3)
+    s2.reset(np.array([[13.5, 8.5, 8.5]]))
+    s2.k_weights(); s2.k_g2p()
+    assert s2.pos4[0, 0] == s2.params.hard_max_x and s2.vel4[0, 0] == 0.0
+
+
+# ---- whole step -------------------------------------------------------------
+def test_free_fall_single_step():
+    """A lone particle far from walls: velocity gains g*dt in one step.
+    (Its mass spreads over 27 cells, which all see the same velocity up to
+    the 1e-5 fixed-point truncation of mass and momentum, relative error
+    ~27e-5 of |v|.)"""
+    s = single((24, 40, 24))
+    s.reset(np.array([[12.5, 30.5, 12.5]]), vel=np.array([[0.2, 0.0, -0.1]]))
+    s.step(1)
+    gdt = s.params.gravity * s.params.dt
+    assert np.allclose(s.vel4[0, :3], [0.2, gdt, -0.1], atol=3e-4 * 0.3)
+    assert np.allclose(s.pos4[0, :3], [12.5 + 0.3 * 0.2, 30.5 + 0.3 * gdt, 12.5 - 0.03],
+                       atol=1e-4)
+
+
+def test_step_permutation_invariance_and_mass():
+    rng = np.random.default_rng(11)
+    from mlsmpm import scenes
+    pos = scenes.lattice_block((2, 2, 2), (10, 12, 10), 0.1, rng)
+    perm = rng.permutation(len(pos))
+    a, b = O.OracleScene(grid=(20, 20, 20)), O.OracleScene(grid=(20, 20, 20))
+    a.reset(pos)
+    b.reset(pos[perm])
+    for _ in range(6):
+        a.step(1)
+        b.step(1)
+        assert np.array_equal(a.cell_mass, b.cell_mass)
+        assert np.array_equal(a.cell_mv, b.cell_mv)
+        assert np.array_equal(a.pos4[perm], b.pos4)
+        n = len(pos)
+        assert n * 1e5 - 27 * n <= a.cell_mass.astype(np.int64).sum() <= n * 1e5
+
+
+def test_fp32_against_fp64_twin_one_step():
+    """One step of a dam break, f32 restatement vs its f64 twin: the f32 error
+    is stated here and must stay below the 1e-5 relative per-step bar."""
+    from mlsmpm import scenes
+    grid, pos, _ = (32, 32, 32), scenes.lattice_block((2, 2, 2), (14, 20, 14), 0.1,
+                                                      np.random.default_rng(2)), None
+    s32, s64 = O.OracleScene(grid=grid), O.OracleScene(grid=grid, real="f64")
+    s32.reset(pos)
+    s64.reset(pos.astype(np.float64))
+    s32.step(1)
+    s64.step(1)
+    rel_pos = np.abs(s32.pos4[:, :3] - s64.pos4[:, :3]).max() / np.abs(s64.pos4[:, :3]).max()
+    vscale = np.abs(s64.vel4[:, :3]).max()
+    rel_vel = np.abs(s32.vel4[:, :3] - s64.vel4[:, :3]).max() / vscale
+    assert rel_pos < 1e-6, rel_pos
+    assert rel_vel < 1e-5, rel_vel
+
+
+# ---- diffuse (shaders.js:707-997) ------------------------------------------
+def test_xorshift_known_answer():
+    # Marsaglia xorshift32 (13, 17, 5) from seed 1 yields 270369
+    assert O.diffuse_rand(1) == np.float32(270369) / np.float32(4294967296.0)
+    assert O.diffuse_rand(0) == 0.0
+
+
+def _pool_scene(D=8, grid=(16, 16, 16)):
+    s = O.OracleScene(grid=grid, diffuse_max=D)
+    return s
+
+
+def test_generate_spawns_spray_and_foam():
+    s = _pool_scene()
+    s.reset(np.array([[8.5, 8.5, 8.5], [6.5, 9.5, 7.5]]), vel=np.array([[3.0, 0.0, 0.0],
+                                                                        [0.0, 4.0, 0.0]]))
+    s.density[:] = [1.0, 3.0]          # spray (< 1.5) and foam (1.5 .. 6)
+    s.gen.seed = 5
+    s.k_generate()
+    assert s.top[0] == 6
+    spray = s.pool[7]                   # first pop takes free_list[top-1] = 7
+    foam = s.pool[6]
+    assert spray["alive"] == 1 and spray["ptype"] == 0 and spray["lifetime"] == 999.0
+    assert foam["alive"] == 1 and foam["ptype"] == 1
+    assert foam["lifetime"] == 3.0      # 3 * min(1, score / 0.5), score >= 0.5
+    assert np.array_equal(spray["pos"], np.array([8.5, 8.5, 8.5, 0], F32))
+    # velocity jitter within +/- 0.15 * speed per axis
+    assert np.all(np.abs(spray["vel"][:3] - [3.0, 0, 0]) <= 0.15 * 3.0 + 1e-6)
+    assert np.all(np.abs(foam["vel"][:3] - [0, 4.0, 0]) <= 0.15 * 4.0 + 1e-6)
+
+
+def test_generate_respects_density_window_and_pool_exhaustion():
+    s = _pool_scene(D=1)
+    s.reset(np.array([[8.5, 8.5, 8.5]] * 3), vel=np.array([[3.0, 0, 0]] * 3))
+    s.density[:] = [7.0, 0.0, 2.0]     # above bubble_min, empty, valid
+    s.k_generate()
+    assert s.top[0] == 0 and s.pool[0]["alive"] == 1 and s.pool[0]["ptype"] == 1
+    s.k_generate()                       # pool exhausted: nothing, top stays 0
+    assert s.top[0] == 0
+
+
+def test_advect_spray_foam_bubble_and_kill():
+    s = _pool_scene(D=6)
+    s.reset(np.zeros((0, 3)))
+    dt, g = s.params.dt, s.params.gravity
+    recs = np.zeros(6, s.L.drec_dtype)
+    recs["alive"] = 1
+    recs[0]["pos"][:3], recs[0]["vel"][:3], recs[0]["ptype"] = [8, 8, 8], [1, 2, 0], 0
+    recs[1]["pos"][:3], recs[1]["ptype"], recs[1]["lifetime"] = [8, 8, 8], 1, 1.0
+    recs[2]["pos"][:3], recs[2]["ptype"], recs[2]["lifetime"] = [8, 8, 8], 1, 0.2
+    recs[3]["pos"][:3], recs[3]["vel"][:3], recs[3]["ptype"] = [8, 8, 8], [1, 0, 0], 2
+    recs[4]["pos"][:3], recs[4]["vel"][:3], recs[4]["ptype"] = [8, 2.1, 8], [0, -1, 0], 0
+    recs[5]["pos"][:3], recs[5]["vel"][:3], recs[5]["ptype"] = [13.9, 8, 8], [1, 0, 0], 0
+    s.set_pool(recs, np.arange(6, dtype=np.uint32), 0)
+    s.cell_v[:] = 0
+    s.k_advect()
+    p = s.pool
+    # spray: ballistic with gravity
+    assert np.allclose(p[0]["vel"][:3], [1, 2 + g * dt, 0], atol=1e-6)
+    assert np.allclose(p[0]["pos"][:3], [8 + dt, 8 + dt * (2 + g * dt), 8], atol=1e-6)
+    # foam: advected by the (zero) grid, lifetime decremented; second one dies
+    assert abs(p[1]["lifetime"] - (1.0 - dt)) < 1e-6 and p[1]["alive"] == 1
+    assert p[2]["alive"] == 0 and p[2]["lifetime"] == np.float32(0.2)   # payload kept
+    # bubble: drag toward zero grid velocity plus buoyancy 1.2 * -g
+    assert np.allclose(p[3]["vel"][:3], [1 + dt * 4 * (0 - 1), dt * (1.2 * -g), 0], atol=1e-6)
+    # floor kill below hard_min
+    assert p[4]["alive"] == 0
+    # side wall reflection with restitution 0.3: x = 14.2 -> 14 - 0.2 = 13.8
+    assert abs(p[5]["pos"][0] - (14.0 - (13.9 + dt - 14.0))) < 1e-5
+    assert abs(p[5]["vel"][0] - (-0.3)) < 1e-6
+    # kills pushed in slot order
+    assert s.top[0] == 2 and list(s.free_list[:2]) == [2, 4]
+
+
+def test_pool_conservation_over_steps():
+    from mlsmpm import scenes
+    pos = scenes.lattice_block((2, 2, 2), (12, 18, 12), 0.1, np.random.default_rng(4))
+    s = O.OracleScene(grid=(28, 28, 28), diffuse_max=5000)
+    s.reset(pos)
+    for _ in range(20):
+        s.step(1)
+        alive = int(s.pool["alive"].sum())
+        assert alive + int(s.top[0]) == 5000
+        free = np.sort(s.free_list[: s.top[0]])
+        dead = np.sort(np.nonzero(s.pool["alive"] == 0)[0]).astype(np.uint32)
+        assert np.array_equal(free, dead)
+
+
+# ---- scenes and literals ----------------------------------------------------
+def test_lattice_counts_match_config_geometry():
+    from mlsmpm import scenes
+    assert len(scenes.dam_break_small()[1]) == 56 * 76 * 56 == 238336
+    rng = np.random.default_rng(0)
+    assert len(scenes.lattice_block((2, 2, 2), (104, 43.5, 126), 0.0, rng)) == 204 * 83 * 248
+
+
+def test_reference_float_literals_round_once():
+    """WGSL evaluates these f32 literals from abstract floats; parsing them
+    straight to f32 gives the same value (no double-rounding case)."""
+    lits = ["0.5", "0.75", "100000.0", "20000.0", "0.15", "0.7", "-0.1", "4.0", "1e-8", "0.85",
+            "0.9", "3.0", "1e-6", "1e-4", "2.0", "1.5", "0.3", "999.0", "4294967295.0", "0.98",
+            "0.01", "8.0", "1.2"]
+    for t in lits:
+        assert np.float32(t) == np.float32(np.float64(t)), t
