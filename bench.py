@@ -108,11 +108,17 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--workload", default="dam-large", choices=["dam-large", "dam-small", "two-block"])
-    ap.add_argument("--sort-period", type=int, default=8)
+    ap.add_argument("--sort-period", type=int, default=1)
     ap.add_argument("--flags", type=int, default=0)
+    ap.add_argument("--tile-ppt", type=int, default=0)
+    ap.add_argument("--tile-margin", type=int, default=0)
+    ap.add_argument("--tile-threads", type=int, default=0)
+    ap.add_argument("--g2p-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-events", action="store_true",
+                    help="time the region by wall clock only (no per-stage events)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -125,7 +131,9 @@ def main() -> int:
 
     grid, pos, vel, desc = workload(args.workload, args.seed)
     n = len(pos)
-    cfg = mlsmpm.default_config(*grid, n, sort_period=args.sort_period, flags=args.flags, device=0)
+    cfg = mlsmpm.default_config(*grid, n, sort_period=args.sort_period, flags=args.flags, device=0,
+                                tile_ppt=args.tile_ppt, tile_margin=args.tile_margin,
+                                tile_threads=args.tile_threads, g2p_threads=args.g2p_threads)
     s = mlsmpm.Scene(cfg)
     s.reset(pos, vel)
     print(f"[bench] {desc}: {n} particles, warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -133,7 +141,7 @@ def main() -> int:
     s.sync()
     g0 = int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"]))
 
-    s.enable_timing(True)
+    s.enable_timing(not args.no_events)
     s.sync()
     t0 = time.perf_counter()
     s.step(args.steps)
@@ -142,6 +150,7 @@ def main() -> int:
     stage = s.stage_times()
     step_ms = s.step_times()
     g1 = int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"]))
+    tstats = s.tile_stats()
     s.enable_timing(False)
     elapsed = t1 - t0
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
@@ -199,6 +208,7 @@ def main() -> int:
         },
         "step_roofline_frac": step_alg / (step_ms_mean * 1e-3) / HBM_PEAK,
         "stage_ms": ms,
+        "tile_stats": tstats,
         "step_latency_ms": {
             "p50": float(np.percentile(step_ms, 50)) if len(step_ms) else None,
             "p99": float(np.percentile(step_ms, 99)) if len(step_ms) else None,

@@ -41,7 +41,7 @@ extern "C" {
 #define MPM_FLAG_KEEP_GRID 1u  /* keep the i32 grids readable after a step
                                   (separate clear pass instead of the fused one) */
 #define MPM_FLAG_NO_LDS 2u     /* force the global-atomic scatter path */
-#define MPM_FLAG_DENSE_GRID 4u /* update every cell, not only active tiles */
+#define MPM_FLAG_NO_GRAPH 4u  /* enqueue every kernel eagerly (no captured HIP graph) */
 
 /* Scene configuration.  Physics fields and defaults follow the FluidGPU
  * constructor (fluid-gpu.js:10-45) and the Params uniform (shaders.js:27-49);
@@ -59,9 +59,11 @@ typedef struct mpm_config {
     float ke_threshold, pressure_threshold, crest_dot_threshold, spawn_rate_k;
     float lifetime_foam, buoyancy, drag, density_spray_max, density_bubble_min;
     uint32_t seed; /* DiffuseGPU._seed initial value; advanced before each generate */
-    uint32_t tile_slices; /* particles per lane in a P2G chunk (4/8/16/32); 0 = default 16 */
-    uint32_t tile_wmax;   /* LDS window budget in cells (27..4096); 0 = default 3072 */
-    uint32_t reserved[4];
+    uint32_t tile_ppt;     /* work-item size in units of 256 particles (4/8/16); 0 = 8 */
+    uint32_t tile_margin;  /* P2G LDS window drift margin in cells (1/2); 0 = 2 */
+    uint32_t tile_threads; /* P2G threads per work item (256); 0 = 256 */
+    uint32_t g2p_threads;  /* G2P threads per work item (256); 0 = 256 */
+    uint32_t reserved[2];
 } mpm_config;
 
 typedef struct mpm_scene mpm_scene;
@@ -76,7 +78,7 @@ const char *mpm_last_error(void);
  * GRAVITY -0.98, REST_DENSITY 8, STIFFNESS 100, EOS_POWER 1, VISCOSITY 0.01,
  * HARD_MIN 2, HARD_MAX_* = grid - 2, WALL_MIN 3, WALL_STIFFNESS 1,
  * LOOKAHEAD_K 2 (fluid-gpu.js:24-45) and the DiffuseGPU tunables
- * (diffuse-gpu.js:20-28), seed 1, sort_period 8. */
+ * (diffuse-gpu.js:20-28), seed 1, sort_period 1 (the re-sort rides on G2P). */
 int mpm_config_default(mpm_config *cfg, uint32_t gx, uint32_t gy, uint32_t gz,
                        uint32_t capacity);
 
@@ -140,8 +142,9 @@ int mpm_diffuse_alive(mpm_scene *s, uint32_t *alive);
 
 /* Replaces assignments to the public tunables of FluidGPU / DiffuseGPU
  * (fluid-gpu.js:24-45, diffuse-gpu.js:20-28): physics and diffuse fields of
- * cfg take effect from the next step.  Grid size, capacities and device are
- * fixed at create (MPM_ERR_INVALID if they differ). */
+ * cfg take effect from the next step.  Grid size, capacities, device, flags,
+ * sort period and tile shape are fixed at create (MPM_ERR_INVALID if they
+ * differ). */
 int mpm_set_params(mpm_scene *s, const mpm_config *cfg);
 int mpm_get_config(const mpm_scene *s, mpm_config *cfg);
 
@@ -165,6 +168,11 @@ int mpm_get_stream(mpm_scene *s, void **hip_stream);
 int mpm_enable_timing(mpm_scene *s, int enable);
 int mpm_stage_times(mpm_scene *s, float *ms7);
 int mpm_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count);
+
+/* Diagnostics of the LDS-staged scatter: out[0] = work items of the current
+ * particle order, out[1] = 8x8x8 bricks in the grid, out[2] = particles,
+ * out[3] = largest item in particles.  Synchronises. */
+int mpm_tile_stats(mpm_scene *s, uint64_t out[4]);
 
 #ifdef __cplusplus
 }

@@ -53,7 +53,7 @@ int mpm_config_default(mpm_config *c, uint32_t gx, uint32_t gy, uint32_t gz, uin
   c->grid_z = gz;
   c->capacity = capacity;
   c->diffuse_capacity = 0;
-  c->sort_period = 8;
+  c->sort_period = 1;
   c->flags = 0;
   c->device = 0;
   // FluidGPU constructor (fluid-gpu.js:24-45)
@@ -182,13 +182,18 @@ int mpm_set_params(mpm_scene *s, const mpm_config *cfg) {
   const mpm_config &o = s->cfg;
   if (cfg->grid_x != o.grid_x || cfg->grid_y != o.grid_y || cfg->grid_z != o.grid_z ||
       cfg->capacity != o.capacity || cfg->diffuse_capacity != o.diffuse_capacity ||
-      cfg->device != o.device)
-    return fail(MPM_ERR_INVALID, "mpm_set_params: grid, capacities and device are fixed");
+      cfg->device != o.device || cfg->flags != o.flags || cfg->sort_period != o.sort_period ||
+      cfg->tile_ppt != o.tile_ppt || cfg->tile_margin != o.tile_margin ||
+      cfg->tile_threads != o.tile_threads || cfg->g2p_threads != o.g2p_threads)
+    return fail(MPM_ERR_INVALID,
+                "mpm_set_params: grid, capacities, device, flags, sort period and tile shape "
+                "are fixed at create");
   int rc = validate_config(*cfg);
   if (rc) return rc;
   uint32_t seed = s->cfg.seed;
   s->cfg = *cfg;
   s->cfg.seed = seed;
+  scene_drop_graph(s);
   return MPM_OK;
 }
 
@@ -227,6 +232,11 @@ int mpm_stage_times(mpm_scene *s, float *ms7) {
 int mpm_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count) {
   if (!s) return fail(MPM_ERR_INVALID, "mpm_step_times: scene is NULL");
   return scene_step_times(s, ms, cap, count);
+}
+
+int mpm_tile_stats(mpm_scene *s, uint64_t out[4]) {
+  if (!s || !out) return fail(MPM_ERR_INVALID, "mpm_tile_stats: NULL argument");
+  return scene_tile_stats(s, out);
 }
 
 }  // extern "C"

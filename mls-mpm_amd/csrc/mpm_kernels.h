@@ -18,21 +18,16 @@ struct DiffuseBufs {
   uint32_t *total;     // scan total
 };
 
-// Tiled path state (mpm_tiled.hip).
-struct TileBufs {
-  int32_t *boxes;  // per chunk: window origin x0,y0,z0 and extent wx,wy,wz (wx = 0: fallback)
-};
-
 // Sort buffers (mpm_tiled.hip).
 struct SortBufs {
-  uint32_t *keys;       // per particle: cell key
-  uint32_t *rank;       // per particle: rank within its key (from the histogram atomics)
-  uint32_t *counts;     // per key
-  uint32_t *offsets;    // per key, exclusive scan of counts
+  uint32_t *keys;       // per particle: Morton rank of its new brick
+  uint32_t *rank;       // per particle: rank among its brick's particles
+  uint32_t *counts;     // per brick rank
+  uint32_t *offsets;    // per brick rank, exclusive scan of counts
   uint32_t *scratch;    // scan scratch
-  uint32_t *tile_rank;  // 4x4x4 grid tile -> position in Morton order
+  uint32_t *brick_rank; // 8x8x8 grid brick -> position in Morton order
   uint32_t *src_of;     // per new slot: old slot (G2P re-layout)
-  uint32_t nkeys;       // 64 * number of tiles
+  uint32_t nkeys;       // number of bricks
 };
 
 void launch_clear_grid(const GState &g, uint32_t G, hipStream_t st);

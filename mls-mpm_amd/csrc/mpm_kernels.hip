@@ -248,6 +248,8 @@ __global__ __launch_bounds__(256) void k_gen_apply(Params P, PState S, GState g,
   }
 }
 
+__global__ void k_zero_total(uint32_t *p) { *p = 0u; }
+
 // top -= min(spawners, top); the step's seed becomes the last-used seed.
 __global__ void k_gen_finish(int32_t *topp, const uint32_t *total, uint32_t *seedp) {
   int32_t top = *topp;
@@ -584,7 +586,7 @@ void launch_diffuse(const Params &P, const PState &S, const GState &g, const Wal
     k_gen_apply<<<blocks(nwg, 256), 256, 0, st>>>(P, S, g, D.seed, D.genbits, D.prefix, D.gensrc,
                                                   D.free_list, D.top, D.pool, nwg);
   } else {
-    (void)hipMemsetAsync(D.total, 0, sizeof(uint32_t), st);
+    k_zero_total<<<1, 1, 0, st>>>(D.total);
   }
   k_gen_finish<<<1, 1, 0, st>>>(D.top, D.total, D.seed);
   k_advect<<<blocks(P.dmax, 256), 256, 0, st>>>(P, g, D.pool, D.killbits, W);

@@ -10,14 +10,34 @@ namespace mpm {
 // momentum contribution per stencil cell (WGSL_P2G_MOM, shaders.js:286-370).
 // `mass_at(k,j,i)` returns the decoded cell mass of stencil cell (i,j,k);
 // `emit(k,j,i,ex,ey,ez)` receives the three encoded contributions.
+// Particle inputs of P2G momentum, loaded ahead of their use so a loop can
+// prefetch the next particle while computing the current one.
+struct MomIn {
+  float px, py, pz, vx, vy, vz, m;
+  float a[9];
+};
+template <bool UNIT>
+__device__ __forceinline__ void load_mom_in(const PState &S, uint32_t p, MomIn &in) {
+  in.px = S.at(PX, p);
+  in.py = S.at(PY, p);
+  in.pz = S.at(PZ, p);
+  in.vx = S.at(VX, p);
+  in.vy = S.at(VY, p);
+  in.vz = S.at(VZ, p);
+  in.m = UNIT ? 1.0f : S.at(PM, p);
+#pragma unroll
+  for (int k = 0; k < 9; k++) in.a[k] = S.at(C00 + k, p);
+}
+
+// Returns the particle density; contributions go to emit() unless the
+// density is below the reference's 1e-8 cut-off.
 template <bool UNIT, class MassAt, class Emit>
-__device__ __forceinline__ void p2g_mom_particle(const Params &P, const PState &S, uint32_t p,
-                                                 const PW &pw, MassAt mass_at, Emit emit) {
-  float vx = S.at(VX, p), vy = S.at(VY, p), vz = S.at(VZ, p);
-  float m = UNIT ? 1.0f : S.at(PM, p);
-  float a00 = S.at(C00, p), a01 = S.at(C01, p), a02 = S.at(C02, p);
-  float a10 = S.at(C10, p), a11 = S.at(C11, p), a12 = S.at(C12, p);
-  float a20 = S.at(C20, p), a21 = S.at(C21, p), a22 = S.at(C22, p);
+__device__ __forceinline__ float p2g_mom_body(const Params &P, const MomIn &in, const PW &pw,
+                                              MassAt mass_at, Emit emit) {
+  const float vx = in.vx, vy = in.vy, vz = in.vz, m = in.m;
+  const float a00 = in.a[0], a01 = in.a[1], a02 = in.a[2];
+  const float a10 = in.a[3], a11 = in.a[4], a12 = in.a[5];
+  const float a20 = in.a[6], a21 = in.a[7], a22 = in.a[8];
 
   float density = 0.0f;
 #pragma unroll
@@ -27,8 +47,7 @@ __device__ __forceinline__ void p2g_mom_particle(const Params &P, const PState &
 #pragma unroll
       for (int i = 0; i < 3; i++)
         density += ((pw.w[0][i] * pw.w[1][j]) * pw.w[2][k]) * mass_at(k, j, i);
-  S.density[p] = density;
-  if (density <= 1e-8f) return;
+  if (density <= 1e-8f) return density;
 
   float rest = P.rest_density;
   float vol = m / fmaxf(density, rest * 0.15f);
@@ -89,6 +108,16 @@ __device__ __forceinline__ void p2g_mom_particle(const Params &P, const PState &
       }
     }
   }
+  return density;
+}
+
+// P2G momentum of particle p, reading its inputs and writing its density.
+template <bool UNIT, class MassAt, class Emit>
+__device__ __forceinline__ void p2g_mom_particle(const Params &P, const PState &S, uint32_t p,
+                                                 const PW &pw, MassAt mass_at, Emit emit) {
+  MomIn in;
+  load_mom_in<UNIT>(S, p, in);
+  S.density[p] = p2g_mom_body<UNIT>(P, in, pw, mass_at, emit);
 }
 
 // Grid update of one cell (WGSL_UPDATE_GRID, shaders.js:443-512).  Returns

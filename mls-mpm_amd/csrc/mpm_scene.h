@@ -21,14 +21,16 @@ struct mpm_scene {
   int unit_mass = 1;
   mpm::GState g{};
   mpm::DiffuseBufs d{};
-  mpm::TileBufs t{};
+  mpm::BrickWork bw{};
+  uint4 *items = nullptr;
+  uint32_t *nitems = nullptr;
+  uint32_t *brick_of_rank = nullptr;
   mpm::SortBufs sort{};
   mpm::Walls walls{};
   uint32_t nwalls = 0;
   uint32_t since_sort = 0;  // steps since the particles were last sorted
   bool sorted_once = false; // particles sorted at least once since reset
-  uint32_t tdx = 0, tdy = 0, tdz = 0;  // 4x4x4 sort tiles per axis
-  mpm::TileCfg tc{16, 3072};
+  mpm::TileCfg tc{8, 2, 256, 256};
   void *stage = nullptr;    // device staging for reads/writes
   size_t stage_bytes = 0;
   // Stage timing: per step, events bracket sort, P2G mass, P2G momentum,
@@ -41,6 +43,18 @@ struct mpm_scene {
   double stage_sum[7] = {};      // ms: p2g_mass, p2g_mom, update, g2p, diffuse, sort, total
   uint64_t timed_steps = 0;
   std::vector<float> step_ms;    // per-step totals since timing was enabled
+  // Captured multi-step graphs, valid when a step starts from (graph_cur,
+  // graph_since).  Untimed: graph[0] only.  Timed: graph[0] and graph[1]
+  // record their steps' events into the two halves of ev[] and are replayed
+  // alternately, so one replay's events are read while the other runs.
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  bool graph_timed = false;
+  int graph_next = 0;            // timed: which graph replays next
+  int graph_pending = -1;        // timed: replay whose events are not yet folded
+  int graph_cur = 0;
+  uint32_t graph_since = 0, graph_len = 0;
+  int post_cur = 0;
+  uint32_t post_since = 0;
   size_t bytes = 0;         // device bytes held by the scene
   void *owned[64] = {};     // allocations to free
   int nowned = 0;
@@ -63,6 +77,8 @@ int scene_set_seed(mpm_scene *s, uint32_t seed);
 int scene_set_walls(mpm_scene *s, const float *wallp);
 int scene_enable_timing(mpm_scene *s, bool on);
 int scene_stage_times(mpm_scene *s, float *ms7);
+int scene_tile_stats(mpm_scene *s, uint64_t out[4]);
+void scene_drop_graph(mpm_scene *s);
 int scene_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count);
 Params scene_params(const mpm_scene *s);
 size_t scene_footprint(const mpm_config &c);

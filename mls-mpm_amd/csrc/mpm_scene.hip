@@ -38,14 +38,17 @@ int validate_config(const mpm_config &c) {
   uint64_t G = (uint64_t)c.grid_x * c.grid_y * c.grid_z;
   if (G > (1ull << 30)) return failf(MPM_ERR_INVALID, "grid has %llu cells, limit 2^30",
                                      (unsigned long long)G);
-  if (c.capacity > (1u << 30))
-    return failf(MPM_ERR_INVALID, "capacity %u exceeds 2^30", c.capacity);
+  if (c.capacity > (1u << 26))
+    return failf(MPM_ERR_INVALID, "capacity %u exceeds 2^26", c.capacity);
   if (c.diffuse_capacity > (1u << 30))
     return failf(MPM_ERR_INVALID, "diffuse_capacity %u exceeds 2^30", c.diffuse_capacity);
-  if (c.tile_slices && !tiled_slices_supported(c.tile_slices))
-    return failf(MPM_ERR_INVALID, "tile_slices %u: supported 4, 8, 16, 32", c.tile_slices);
-  if (c.tile_wmax && (c.tile_wmax < 27 || c.tile_wmax > 4096))
-    return failf(MPM_ERR_INVALID, "tile_wmax %u outside [27, 4096]", c.tile_wmax);
+  if (c.tile_ppt && !tiled_ppt_supported(c.tile_ppt))
+    return failf(MPM_ERR_INVALID, "tile_ppt %u: supported 4, 8, 16", c.tile_ppt);
+  if (c.tile_margin && !tiled_margin_supported(c.tile_margin))
+    return failf(MPM_ERR_INVALID, "tile_margin %u: supported 1, 2", c.tile_margin);
+  if ((c.tile_threads && !tiled_threads_supported(c.tile_threads)) ||
+      (c.g2p_threads && !tiled_threads_supported(c.g2p_threads)))
+    return failf(MPM_ERR_INVALID, "tile_threads / g2p_threads: supported 256");
   if (c.eos_power != 1.0f)
     return failf(MPM_ERR_UNSUPPORTED, "eos_power %g: only 1 (the reference default) is built",
                  (double)c.eos_power);
@@ -70,14 +73,17 @@ int validate_config(const mpm_config &c) {
 
 // Device bytes a scene of this configuration holds (used for admission).
 size_t scene_footprint(const mpm_config &c) {
-  size_t cap = c.capacity, G = (size_t)c.grid_x * c.grid_y * c.grid_z, D = c.diffuse_capacity;
+  const size_t cap = c.capacity ? c.capacity : 1, D = c.diffuse_capacity;
+  const size_t G = (size_t)c.grid_x * c.grid_y * c.grid_z;
+  const size_t nb = (size_t)cdiv(c.grid_x, 8) * cdiv(c.grid_y, 8) * cdiv(c.grid_z, 8);
+  const size_t nit = nb + cap / 1024 + 1;
   size_t b = 0;
-  b += 2 * cap * (NPLANES * 4 + 4);     // particle planes + ids, double-buffered for sorting
-  b += cap * 4;                         // density
-  b += G * (4 * 4 + 16);                // i32 accumulators + float4 velocity/mass
-  b += cap * 8 + G * 8 + 4096;          // sort keys/ranks, counts/offsets
-  b += cdiv(cap, 1024) * 24 + 2 * cdiv(G, 512);  // chunk boxes, tile flags
-  if (D) b += D * (48 + 4) + cap * 4 + (cap / 32 + D / 32) * 8 + 4096;
+  b += 2 * cap * (NPLANES * 4 + 4 + 4);  // particle planes, ids, density (two orders)
+  b += G * (8 * 4 + 16);                 // i32 grids, overflow grids, float4 velocity/mass
+  b += cap * 12 + nb * 16;               // keys, ranks, src_of; counts, offsets, first, count
+  b += nit * (16 + 4 * 4000);            // work items and ghost cores
+  b += nb * 8 + 65536;                   // flags, lists, scan scratch
+  if (D) b += D * (48 + 4) + cap * 4 + (cap / 32 + D / 32 + 2) * 8 + 4096;
   return b;
 }
 
@@ -162,8 +168,8 @@ static int reset_diffuse(mpm_scene *s) {
   return MPM_OK;
 }
 
-// Rank of every 4x4x4 tile when tiles are visited in Morton (Z) order, so
-// that consecutive sort keys stay spatially close across tile boundaries.
+// Rank of every 8x8x8 brick when bricks are visited in Morton (Z) order, so
+// that consecutive sort keys stay spatially close across brick boundaries.
 static uint64_t spread3(uint32_t v) {
   uint64_t x = v & 0x1fffffu;
   x = (x | x << 32) & 0x1f00000000ffffull;
@@ -186,6 +192,19 @@ static std::vector<uint32_t> morton_tile_rank(uint32_t tx, uint32_t ty, uint32_t
   return rank;
 }
 
+// Zero every grid-sized accumulator and the per-brick flags (create, reset).
+static int zero_grids(mpm_scene *s) {
+  const size_t G = s->G;
+  int32_t *i32[] = {s->g.m, s->g.mx, s->g.my, s->g.mz, s->bw.om, s->bw.ox, s->bw.oy, s->bw.oz};
+  for (int32_t *p : i32)
+    if (p) HIPCHK(hipMemsetAsync(p, 0, G * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->g.v, 0, G * 16, s->stream));
+  uint8_t *u8[] = {s->bw.ofl, s->bw.act, s->bw.occ};
+  for (uint8_t *p : u8)
+    if (p) HIPCHK(hipMemsetAsync(p, 0, s->bw.nbricks, s->stream));
+  return MPM_OK;
+}
+
 int scene_init(mpm_scene *s, const mpm_config &c) {
   s->cfg = c;
   int ndev = 0;
@@ -206,37 +225,68 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
   }
   DALLOC(s->dens[0], cap * 4);
   DALLOC(s->dens[1], cap * 4);
-  // tiled P2G and the sort
-  s->tc.slices = c.tile_slices ? c.tile_slices : 16;
-  s->tc.wmax = c.tile_wmax ? c.tile_wmax : 3072;
-  DALLOC(s->t.boxes, (size_t)cdiv(cap, 256) * 6 * 4);
-  s->tdx = cdiv(c.grid_x, 4);
-  s->tdy = cdiv(c.grid_y, 4);
-  s->tdz = cdiv(c.grid_z, 4);
+  // brick-sorted layout, LDS-staged P2G
+  s->tc.ppt = c.tile_ppt ? c.tile_ppt : 8;
+  s->tc.margin = c.tile_margin ? c.tile_margin : 2;
+  s->tc.threads = c.tile_threads ? c.tile_threads : 256;
+  s->tc.g2p_threads = c.g2p_threads ? c.g2p_threads : 256;
   {
-    const uint32_t ntiles = s->tdx * s->tdy * s->tdz;
-    s->sort.nkeys = ntiles * 64u;
+    BrickWork &bw = s->bw;
+    bw.nbx = cdiv(c.grid_x, 8);
+    bw.nby = cdiv(c.grid_y, 8);
+    bw.nbz = cdiv(c.grid_z, 8);
+    bw.nbricks = bw.nbx * bw.nby * bw.nbz;
+    s->sort.nkeys = bw.nbricks;
+    const uint32_t nit = max_items((uint32_t)cap, bw.nbricks, 4);
+    DALLOC(s->items, (size_t)nit * 16);
+    DALLOC(s->nitems, 4);
+    DALLOC(s->brick_of_rank, (size_t)bw.nbricks * 4);
+    DALLOC(s->sort.brick_rank, (size_t)bw.nbricks * 4);
+    bw.items = s->items;
+    bw.items_w = s->items;
+    bw.nitems = s->nitems;
+    bw.brick_of_rank = s->brick_of_rank;
+    bw.brick_rank = s->sort.brick_rank;
+    DALLOC(bw.first, (size_t)bw.nbricks * 4);
+    DALLOC(bw.count, (size_t)bw.nbricks * 4);
+    const size_t core = 1000;  // 10^3 ghost cells per item
+    DALLOC(bw.gm, (size_t)nit * core * 4);
+    DALLOC(bw.gx, (size_t)nit * core * 4);
+    DALLOC(bw.gy, (size_t)nit * core * 4);
+    DALLOC(bw.gz, (size_t)nit * core * 4);
+    DALLOC(bw.om, G * 4);
+    DALLOC(bw.ox, G * 4);
+    DALLOC(bw.oy, G * 4);
+    DALLOC(bw.oz, G * 4);
+    DALLOC(bw.ofl, bw.nbricks);
+    DALLOC(bw.act, bw.nbricks);
+    DALLOC(bw.occ, bw.nbricks);
+    DALLOC(bw.list, (size_t)bw.nbricks * 4);
+    DALLOC(bw.nlist, 4);
+    HIPCHK(hipMemsetAsync(bw.count, 0, (size_t)bw.nbricks * 4, s->stream));
     DALLOC(s->sort.keys, cap * 4);
     DALLOC(s->sort.rank, cap * 4);
     DALLOC(s->sort.counts, (size_t)s->sort.nkeys * 4);
     DALLOC(s->sort.offsets, (size_t)s->sort.nkeys * 4);
     DALLOC(s->sort.scratch, (size_t)scan_scratch_words(s->sort.nkeys) * 4);
-    DALLOC(s->sort.tile_rank, (size_t)ntiles * 4);
     DALLOC(s->sort.src_of, cap * 4);
     HIPCHK(hipMemsetAsync(s->sort.counts, 0, (size_t)s->sort.nkeys * 4, s->stream));
-    std::vector<uint32_t> rank = morton_tile_rank(s->tdx, s->tdy, s->tdz);
-    HIPCHK(hipMemcpy(s->sort.tile_rank, rank.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(s->nitems, 0, 4, s->stream));
+    std::vector<uint32_t> rank = morton_tile_rank(bw.nbx, bw.nby, bw.nbz);
+    std::vector<uint32_t> inv(rank.size());
+    for (size_t i = 0; i < rank.size(); i++) inv[rank[i]] = (uint32_t)i;
+    HIPCHK(hipMemcpy(s->sort.brick_rank, rank.data(), rank.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->brick_of_rank, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
   }
   DALLOC(s->g.m, G * 4);
   DALLOC(s->g.mx, G * 4);
   DALLOC(s->g.my, G * 4);
   DALLOC(s->g.mz, G * 4);
   DALLOC(s->g.v, G * 16);
-  HIPCHK(hipMemsetAsync(s->g.m, 0, G * 4, s->stream));
-  HIPCHK(hipMemsetAsync(s->g.mx, 0, G * 4, s->stream));
-  HIPCHK(hipMemsetAsync(s->g.my, 0, G * 4, s->stream));
-  HIPCHK(hipMemsetAsync(s->g.mz, 0, G * 4, s->stream));
-  HIPCHK(hipMemsetAsync(s->g.v, 0, G * 16, s->stream));
+  {
+    int rz = zero_grids(s);
+    if (rz) return rz;
+  }
   const uint32_t D = c.diffuse_capacity;
   if (D) {
     uint32_t nwg = cdiv(cap, 32), nwk = cdiv(D, 32);
@@ -264,6 +314,7 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
 
 void scene_free(mpm_scene *s) {
   if (!s) return;
+  scene_drop_graph(s);
   if (s->stream) {
     (void)hipSetDevice(s->cfg.device);
     (void)hipStreamSynchronize(s->stream);
@@ -326,6 +377,7 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
   }
   HIPCHK(hipSetDevice(c.device));
   HIPCHK(hipStreamSynchronize(s->stream));
+  scene_drop_graph(s);
   s->n = n;
   s->cur = 0;
   s->steps = 0;
@@ -360,113 +412,262 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
     launch_import_particles(S, n, dp, dv, dc, dm, s->stream);
     HIPCHK(hipGetLastError());
   }
-  const size_t G = s->G;
-  HIPCHK(hipMemsetAsync(s->g.m, 0, G * 4, s->stream));
-  HIPCHK(hipMemsetAsync(s->g.mx, 0, G * 4, s->stream));
-  HIPCHK(hipMemsetAsync(s->g.my, 0, G * 4, s->stream));
-  HIPCHK(hipMemsetAsync(s->g.mz, 0, G * 4, s->stream));
-  HIPCHK(hipMemsetAsync(s->g.v, 0, G * 16, s->stream));
-  int rc = reset_diffuse(s);
+  int rc = zero_grids(s);
+  if (rc) return rc;
+  rc = reset_diffuse(s);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
   return MPM_OK;
 }
 
-// Fold the recorded per-step events into the stage sums.
-static int fold_timing(mpm_scene *s) {
+// Add one step's 7 events (start, sort, P2G mass, P2G momentum, update, G2P,
+// diffuse) to the stage sums.
+static int fold_step_events(mpm_scene *s, hipEvent_t *e) {
+  const int E = mpm_scene::kEvPerStep;
+  float ms[6];
+  for (int i = 0; i < 6; i++) HIPCHK(hipEventElapsedTime(&ms[i], e[i], e[i + 1]));
+  float tot;
+  HIPCHK(hipEventElapsedTime(&tot, e[0], e[E - 1]));
+  s->stage_sum[5] += ms[0];
+  s->stage_sum[0] += ms[1];
+  s->stage_sum[1] += ms[2];
+  s->stage_sum[2] += ms[3];
+  s->stage_sum[3] += ms[4];
+  s->stage_sum[4] += ms[5];
+  s->stage_sum[6] += tot;
+  s->step_ms.push_back(tot);
+  s->timed_steps++;
+  return MPM_OK;
+}
+
+static int fold_graph_events(mpm_scene *s);
+
+// Fold the steps recorded by eager enqueues.
+static int fold_eager(mpm_scene *s) {
   const int E = mpm_scene::kEvPerStep;
   for (int k = 0; k < s->ev_used; k++) {
     hipEvent_t *e = &s->ev[k * E];
     HIPCHK(hipEventSynchronize(e[E - 1]));
-    float ms[6];
-    for (int i = 0; i < 6; i++) HIPCHK(hipEventElapsedTime(&ms[i], e[i], e[i + 1]));
-    float tot;
-    HIPCHK(hipEventElapsedTime(&tot, e[0], e[E - 1]));
-    // event order: start, sort, p2g_mass, p2g_mom, update, g2p, diffuse
-    s->stage_sum[5] += ms[0];
-    s->stage_sum[0] += ms[1];
-    s->stage_sum[1] += ms[2];
-    s->stage_sum[2] += ms[3];
-    s->stage_sum[3] += ms[4];
-    s->stage_sum[4] += ms[5];
-    s->stage_sum[6] += tot;
-    s->step_ms.push_back(tot);
-    s->timed_steps++;
+    int rc = fold_step_events(s, e);
+    if (rc) return rc;
   }
   s->ev_used = 0;
   return MPM_OK;
 }
 
+// Fold every recorded step (eager pool and pending graph replay).
+static int fold_timing(mpm_scene *s) {
+  int rc = fold_eager(s);
+  if (rc) return rc;
+  return fold_graph_events(s);
+}
+
+// Enqueue one step.  e (optional) receives 7 events: start, after the sort,
+// P2G mass, P2G momentum, grid update, G2P, diffuse.
+// When captured (graph_events), the events become event-record nodes.
+static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) {
+  const bool keep = (s->cfg.flags & MPM_FLAG_KEEP_GRID) != 0;
+  Params P = scene_params(s);
+  PState S = pstate(s, s->cur);
+  hipStream_t st = s->stream;
+  auto mark = [&](int i) -> int {
+    if (e) {
+      if (graph_events) HIPCHK(hipEventRecordWithFlags(e[i], st, hipEventRecordExternal));
+      else HIPCHK(hipEventRecord(e[i], st));
+    }
+    return MPM_OK;
+  };
+  int rc = mark(0);
+  if (rc) return rc;
+  const uint32_t period = s->cfg.sort_period;
+  // The LDS-staged scatter works on brick-sorted particles; without a sort
+  // period the direct per-particle form runs.
+  const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS) && period != 0;
+  // Layout policy: the first step after a reset sorts the caller-order
+  // particles with a full sort; afterwards every `period` steps the P2G mass
+  // pass records new bricks and G2P writes the new state in sorted order.
+  // The direct path has no key recording and sorts with the full sort.
+  bool relayout = false;
+  if (period && s->n) {
+    bool due = !s->sorted_once || s->since_sort >= period;
+    if (due && (!tiled || !s->sorted_once)) {
+      PState dst = pstate(s, s->cur ^ 1);
+      launch_sort(P, S, dst, s->sort, s->bw, s->tc, st);
+      s->cur ^= 1;
+      S = dst;
+      s->since_sort = 0;
+      s->sorted_once = true;
+    } else if (due) {
+      relayout = true;
+    }
+  }
+  if ((rc = mark(1))) return rc;
+  if (keep && !tiled) launch_clear_grid(s->g, s->G, st);
+  KeyOut ko{};
+  if (relayout) ko = key_out(s->sort, s->bw);
+  if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, s->bw, ko, st);
+  else launch_p2g_mass_global(P, S, s->g, st);
+  if ((rc = mark(2))) return rc;
+  if (tiled) launch_p2g_mom_tiled(P, S, s->g, s->tc, s->bw, st);
+  else launch_p2g_mom_global(P, S, s->g, st);
+  if ((rc = mark(3))) return rc;
+  if (tiled) launch_resolve_update(P, s->g, s->tc, s->bw, s->walls, keep, st);
+  else launch_update_dense(P, s->g, s->walls, !keep, st);
+  if ((rc = mark(4))) return rc;
+  if (relayout) {
+    launch_relayout_prep(P, s->sort, s->bw, s->tc, st);
+    PState dst = pstate(s, s->cur ^ 1);
+    launch_g2p_tiled(P, S, dst, s->g, s->walls, s->tc, s->bw, s->sort.src_of, st);
+    s->cur ^= 1;
+    S = dst;
+    s->since_sort = 0;
+  } else if (tiled) {
+    launch_g2p_tiled(P, S, S, s->g, s->walls, s->tc, s->bw, nullptr, st);
+  } else {
+    launch_g2p_global(P, S, s->g, s->walls, st);
+  }
+  if ((rc = mark(5))) return rc;
+  launch_diffuse(P, S, s->g, s->walls, s->d, st);
+  if ((rc = mark(6))) return rc;
+  HIPCHK(hipGetLastError());
+  s->steps++;
+  s->since_sort++;
+  return MPM_OK;
+}
+
+void scene_drop_graph(mpm_scene *s) {
+  for (hipGraphExec_t &g : s->graph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  s->graph_len = 0;
+  s->graph_next = 0;
+  s->graph_pending = -1;
+}
+
+// Steps per captured graph: a whole number of re-sort cycles (the buffer
+// parity flips at every re-sort, so 2 * period steps return to the start).
+static uint32_t graph_steps(uint32_t period) {
+  if (period == 0) return 0;  // direct form: never captured
+  uint32_t cyc = 2 * period;
+  uint32_t k = (16 + cyc - 1) / cyc;
+  return cyc * k;
+}
+
+// Capture graph_steps() steps from the current layout state into graph[slot];
+// timed graphs record their events into ev[slot * L * kEvPerStep ...].
+static int capture_one(mpm_scene *s, int slot, bool timed) {
+  const int cur0 = s->cur;
+  const uint32_t since0 = s->since_sort;
+  const uint64_t steps0 = s->steps;
+  const uint32_t L = graph_steps(s->cfg.sort_period);
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = MPM_OK;
+  for (uint32_t i = 0; i < L && !rc; i++) {
+    hipEvent_t *e = timed ? &s->ev[((size_t)slot * L + i) * mpm_scene::kEvPerStep] : nullptr;
+    rc = enqueue_step(s, e, true);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t err = hipStreamEndCapture(s->stream, &g);
+  s->post_cur = s->cur;
+  s->post_since = s->since_sort;
+  // capture only recorded the work: put the host state back
+  s->cur = cur0;
+  s->since_sort = since0;
+  s->steps = steps0;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (err != hipSuccess) return hip_fail(err, "hipStreamEndCapture");
+  err = hipGraphInstantiate(&s->graph[slot], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (err != hipSuccess) {
+    s->graph[slot] = nullptr;
+    return hip_fail(err, "hipGraphInstantiate");
+  }
+  return MPM_OK;
+}
+
+static int capture_graphs(mpm_scene *s, bool timed) {
+  scene_drop_graph(s);
+  for (int slot = 0; slot < (timed ? 2 : 1); slot++) {
+    int rc = capture_one(s, slot, timed);
+    if (rc) {
+      scene_drop_graph(s);
+      return rc;
+    }
+  }
+  s->graph_timed = timed;
+  s->graph_cur = s->cur;
+  s->graph_since = s->since_sort;
+  s->graph_len = graph_steps(s->cfg.sort_period);
+  return MPM_OK;
+}
+
+// Fold the events of a finished timed replay (waits for it).
+static int fold_graph_events(mpm_scene *s) {
+  if (s->graph_pending < 0) return MPM_OK;
+  const int E = mpm_scene::kEvPerStep;
+  const size_t base = (size_t)s->graph_pending * s->graph_len * E;
+  for (uint32_t k = 0; k < s->graph_len; k++) {
+    hipEvent_t *e = &s->ev[base + (size_t)k * E];
+    HIPCHK(hipEventSynchronize(e[E - 1]));
+    int rc = fold_step_events(s, e);
+    if (rc) return rc;
+  }
+  s->graph_pending = -1;
+  return MPM_OK;
+}
+
 int scene_step(mpm_scene *s, uint32_t nsteps) {
   HIPCHK(hipSetDevice(s->cfg.device));
-  const bool keep = (s->cfg.flags & MPM_FLAG_KEEP_GRID) != 0;
-  for (uint32_t it = 0; it < nsteps; it++) {
-    Params P = scene_params(s);
-    PState S = pstate(s, s->cur);
-    hipStream_t st = s->stream;
-    hipEvent_t *e = nullptr;
-    if (s->timing) {
-      if (s->ev_used == mpm_scene::kEvSteps) {
+  const uint32_t period = s->cfg.sort_period;
+  const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS) && period != 0;
+  while (nsteps > 0) {
+    const uint32_t L = graph_steps(period);
+    const bool graphable = L > 0 && !(s->cfg.flags & MPM_FLAG_NO_GRAPH) && tiled && s->sorted_once &&
+                           s->n > 0 && (!s->timing || 2 * L <= (uint32_t)mpm_scene::kEvSteps);
+    if (graphable && nsteps >= L) {
+      if (!s->graph[0] || s->graph_timed != s->timing || s->graph_cur != s->cur ||
+          s->graph_since != s->since_sort) {
         int rc = fold_timing(s);
         if (rc) return rc;
+        if ((rc = capture_graphs(s, s->timing))) return rc;
+      }
+      int slot = 0;
+      if (s->timing) {
+        // eager-step events share the pool with the graphs' events
+        int rc = fold_eager(s);
+        if (rc) return rc;
+        slot = s->graph_next;
+        HIPCHK(hipGraphLaunch(s->graph[slot], s->stream));
+        // the other graph's replay (if any) was enqueued before this one
+        if ((rc = fold_graph_events(s))) return rc;
+        s->graph_pending = slot;
+        s->graph_next = slot ^ 1;
+      } else {
+        HIPCHK(hipGraphLaunch(s->graph[0], s->stream));
+      }
+      s->cur = s->post_cur;
+      s->since_sort = s->post_since;
+      s->steps += s->graph_len;
+      nsteps -= s->graph_len;
+      continue;
+    }
+    hipEvent_t *e = nullptr;
+    if (s->timing) {
+      int rc = fold_graph_events(s);
+      if (rc) return rc;
+      if (s->ev_used == mpm_scene::kEvSteps) {
+        if ((rc = fold_eager(s))) return rc;
       }
       e = &s->ev[s->ev_used * mpm_scene::kEvPerStep];
       s->ev_used++;
     }
-    auto mark = [&](int i) -> int {
-      if (e) HIPCHK(hipEventRecord(e[i], st));
-      return MPM_OK;
-    };
-    int rc = mark(0);
+    int rc = enqueue_step(s, e);
     if (rc) return rc;
-    const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS);
-    const uint32_t period = s->cfg.sort_period;
-    // Layout policy: the first step after a reset sorts the caller-order
-    // particles with a full sort; afterwards every `period` steps the P2G mass
-    // pass records cell keys and G2P writes the new state in sorted order.
-    // The direct path has no key recording and sorts with the full sort.
-    bool relayout = false;
-    if (period && s->n) {
-      bool due = !s->sorted_once || s->since_sort >= period;
-      if (due && (!tiled || !s->sorted_once)) {
-        PState dst = pstate(s, s->cur ^ 1);
-        launch_sort(P, S, dst, s->sort, s->tdx, s->tdy, s->tc.slices, st);
-        s->cur ^= 1;
-        S = dst;
-        s->since_sort = 0;
-        s->sorted_once = true;
-      } else if (due) {
-        relayout = true;
-      }
-    }
-    if ((rc = mark(1))) return rc;
-    if (keep) launch_clear_grid(s->g, s->G, st);
-    KeyOut ko{};
-    if (relayout) ko = key_out(s->sort, s->tdx, s->tdy);
-    if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, s->t.boxes, ko, st);
-    else launch_p2g_mass_global(P, S, s->g, st);
-    if ((rc = mark(2))) return rc;
-    if (tiled) launch_p2g_mom_tiled(P, S, s->g, s->tc, s->t.boxes, st);
-    else launch_p2g_mom_global(P, S, s->g, st);
-    if ((rc = mark(3))) return rc;
-    launch_update_dense(P, s->g, s->walls, !keep, st);
-    if ((rc = mark(4))) return rc;
-    if (relayout) {
-      launch_relayout_prep(P, s->sort, s->tc.slices, st);
-      PState dst = pstate(s, s->cur ^ 1);
-      launch_g2p_relayout(P, S, dst, s->g, s->walls, s->sort.src_of, st);
-      s->cur ^= 1;
-      S = dst;
-      s->since_sort = 0;
-    } else {
-      launch_g2p_global(P, S, s->g, s->walls, st);
-    }
-    if ((rc = mark(5))) return rc;
-    launch_diffuse(P, S, s->g, s->walls, s->d, st);
-    if ((rc = mark(6))) return rc;
-    HIPCHK(hipGetLastError());
-    s->steps++;
-    s->since_sort++;
+    nsteps--;
   }
   return MPM_OK;
 }
@@ -563,6 +764,7 @@ int scene_set_seed(mpm_scene *s, uint32_t seed) {
 }
 
 int scene_set_walls(mpm_scene *s, const float *wallp) {
+  scene_drop_graph(s);
   if (!wallp) {
     s->nwalls = 0;
     memset(&s->walls, 0, sizeof s->walls);
@@ -583,6 +785,7 @@ int scene_enable_timing(mpm_scene *s, bool on) {
   HIPCHK(hipStreamSynchronize(s->stream));
   s->timing = on;
   s->ev_used = 0;
+  s->graph_pending = -1;
   for (double &v : s->stage_sum) v = 0.0;
   s->timed_steps = 0;
   s->step_ms.clear();
@@ -595,6 +798,27 @@ int scene_stage_times(mpm_scene *s, float *ms7) {
   if (rc) return rc;
   for (int i = 0; i < 7; i++)
     ms7[i] = s->timed_steps ? (float)(s->stage_sum[i] / (double)s->timed_steps) : 0.0f;
+  return MPM_OK;
+}
+
+int scene_tile_stats(mpm_scene *s, uint64_t out[4]) {
+  HIPCHK(hipSetDevice(s->cfg.device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  uint32_t nit = 0;
+  HIPCHK(hipMemcpy(&nit, s->nitems, 4, hipMemcpyDeviceToHost));
+  std::vector<uint4> it(nit);
+  if (nit) HIPCHK(hipMemcpy(it.data(), s->items, (size_t)nit * 16, hipMemcpyDeviceToHost));
+  uint64_t mx = 0, bricks = 0;
+  uint32_t last = 0xffffffffu;
+  for (const uint4 &q : it) {
+    if (q.z > mx) mx = q.z;
+    if (q.x != last) bricks++;
+    last = q.x;
+  }
+  out[0] = nit;
+  out[1] = s->bw.nbricks;
+  out[2] = s->n;
+  out[3] = mx;
   return MPM_OK;
 }
 

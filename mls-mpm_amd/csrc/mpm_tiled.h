@@ -4,29 +4,56 @@
 
 namespace mpm {
 
-// Tile shape of the LDS-staged P2G: chunks of 64*slices particles, windows of
-// at most wmax cells (LDS: 4 B per cell for mass, 16 B per cell for momentum).
+// Shape of the LDS-staged P2G: work items of at most 256 * ppt particles of
+// one 8x8x8 brick, LDS windows of (10 + 2 * margin)^3 cells.
 struct TileCfg {
-  uint32_t slices;  // 4, 8, 16 or 32
-  uint32_t wmax;    // window cells
+  uint32_t ppt;          // item size in units of 256 particles: 4, 8 or 16
+  uint32_t margin;       // drift margin in cells: 1 or 2
+  uint32_t threads;      // P2G threads per item: 256, 512 or 1024
+  uint32_t g2p_threads;  // G2P threads per item: 256, 512 or 1024
 };
 
-// Where the P2G mass pass records each particle's cell key and its rank among
-// the particles of that cell (counts == nullptr: do not record).
+// Work items of the brick-ordered particle arrays.
+struct BrickWork {
+  const uint4 *items;            // (brick rank, first particle, count, 0)
+  uint4 *items_w;                // same buffer, written by the sort
+  uint32_t *nitems;              // device count of items
+  uint32_t *first, *count;       // per brick rank: first item and number of items
+  const uint32_t *brick_of_rank; // brick index of each Morton rank
+  const uint32_t *brick_rank;    // Morton rank of each brick index
+  int32_t *gm;                   // ghost mass cores (10^3 cells), one per item
+  int32_t *gx, *gy, *gz;         // ghost momentum cores, one per item
+  int32_t *om, *ox, *oy, *oz;    // overflow grids (zero outside flagged bricks)
+  uint8_t *ofl;                  // per brick: overflow grid may hold values here
+  uint8_t *act;                  // per brick: bit 0 active this step, bit 1 last step
+  uint8_t *occ;                  // per brick: has particles in the current order
+  uint32_t *list, *nlist;        // bricks to resolve this step
+  uint32_t nbx, nby, nbz, nbricks;
+};
+
+// Where the P2G mass pass records each particle's new brick (Morton rank) and
+// its rank among that brick's particles (counts == nullptr: do not record).
 struct KeyOut {
   uint32_t *counts, *keys, *rank;
-  const uint32_t *tile_rank;
-  uint32_t tdx, tdy;
 };
 
-bool tiled_slices_supported(uint32_t sl);
-KeyOut key_out(const SortBufs &sb, uint32_t tdx, uint32_t tdy);
+bool tiled_ppt_supported(uint32_t ppt);
+bool tiled_margin_supported(uint32_t m);
+bool tiled_threads_supported(uint32_t t);
+uint32_t max_items(uint32_t n, uint32_t nbricks, uint32_t ppt);
+KeyOut key_out(const SortBufs &sb, const BrickWork &bw);
 void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
-                           int32_t *boxes, const KeyOut &ko, hipStream_t st);
+                           const BrickWork &bw, const KeyOut &ko, hipStream_t st);
 void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
-                          const int32_t *boxes, hipStream_t st);
+                          const BrickWork &bw, hipStream_t st);
+void launch_resolve_update(const Params &P, const GState &g, const TileCfg &tc,
+                           const BrickWork &bw, const Walls &W, bool keep, hipStream_t st);
+void launch_g2p_tiled(const Params &P, const PState &S, const PState &D, const GState &g,
+                      const Walls &W, const TileCfg &tc, const BrickWork &bw,
+                      const uint32_t *src_of, hipStream_t st);
 void launch_sort(const Params &P, const PState &src, const PState &dst, const SortBufs &sb,
-                 uint32_t tdx, uint32_t tdy, uint32_t slices, hipStream_t st);
-void launch_relayout_prep(const Params &P, const SortBufs &sb, uint32_t slices, hipStream_t st);
+                 const BrickWork &bw, const TileCfg &tc, hipStream_t st);
+void launch_relayout_prep(const Params &P, const SortBufs &sb, const BrickWork &bw,
+                          const TileCfg &tc, hipStream_t st);
 
 }  // namespace mpm

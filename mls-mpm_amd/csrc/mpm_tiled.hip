@@ -1,307 +1,602 @@
-// mpm_tiled.hip -- the sorted, LDS-staged forms of the P2G scatters and the
-// periodic particle sort.
+// mpm_tiled.hip -- the brick-ordered, LDS-staged P2G / G2P and the particle
+// re-sort.
 //
-// Layout.  Particles are re-sorted by cell every `sort_period` steps with a
-// counting sort whose key orders cells tile-major (4x4x4 tiles in Morton
-// order, x fastest inside a tile).  The sorted sequence is cut into chunks of
-// CH = 64*SL particles; inside a full chunk the particle of sorted rank
-// q = lane*SL + slice is stored at slot chunk*CH + slice*64 + lane.  A wave
-// reading 64 consecutive slots therefore loads coalesced memory while its
-// lanes hold particles SL ranks apart, i.e. in different cells, so the LDS
-// atomics of one instruction rarely hit the same address.  The last, partial
-// chunk keeps sorted order; every slot below n holds a particle.
+// Layout.  Particles are kept grouped by the 8x8x8-cell brick holding them,
+// bricks in Morton order, so every brick's particles are one contiguous range
+// of the particle arrays.  A work item is a piece of one brick's range (at
+// most 256 * ppt particles); inside an item the particles are stored
+// column-major over rows of 256 slots (item_slot), so that a P2G block
+// reading slots k*256 + t (coalesced) holds, in the 64 lanes of one
+// instruction, particles that sit K = ceil(count/256) apart in the item,
+// i.e. mostly in different cells.
 //
-// P2G.  One block per chunk.  The block reduces its particles' base cells to
-// a window (the box their 3x3x3 stencils cover), accumulates the encoded i32
-// contributions of all its particles into that window in LDS (ds_add), and
-// adds every non-zero window cell to the global grid with one i32 atomic.
-// Integer addition is associative, so the grid is bit-identical to per-
-// particle global atomics.  A chunk whose window exceeds the LDS budget
-// (particles far apart, e.g. before the first sort) takes the direct
-// per-particle global-atomic path instead.
-#include <climits>
-
+// Re-sort (every `sort_period` steps, by default every step).  The P2G mass
+// pass counts, per block in LDS, how many of its particles now lie in each of
+// the 27 bricks around its own, reserves ranges of those bricks with one
+// global atomic per brick and stores each particle's (brick, rank).  A scan
+// over the per-brick counts gives brick offsets, the work items of the new
+// order and the inverse permutation; G2P then writes every particle's new
+// state straight into its new slot, so the re-sort moves no extra data.  Only
+// the first step after a reset (caller order) runs a separate gather pass.
+//
+// P2G.  An item's block owns an LDS window: its brick, the 1-cell stencil
+// halo and a drift margin of M cells ((10+2M)^3 cells).  The block adds the
+// encoded i32 contributions of its particles into the window with LDS
+// atomics, then stores the window's core (brick + halo, 10^3 cells) to the
+// item's ghost buffer with plain stores.  Non-zero margin cells, and
+// particles that drifted out of the window altogether, go to an overflow grid
+// through global atomics and flag the bricks they touch.  A resolve pass (one
+// block per brick that can hold mass) sums, for every cell, the ghost cores
+// of the items of the surrounding bricks plus flagged overflow; the momentum
+// resolve also performs the grid update.  Integer addition is associative,
+// so every grid value is bit-identical to per-particle global atomics.
 #include "mpm_kernels.h"
 #include "mpm_ops.h"
 #include "mpm_tiled.h"
 
 namespace mpm {
 
-constexpr int kTB = 256;  // threads per block in the tiled kernels
+constexpr int kTB = 256;    // row width of an item (slots per row)
+constexpr int kCore = 10;   // brick + 1-cell halo per axis
+constexpr int kCoreV = kCore * kCore * kCore;
+constexpr uint32_t kFar = 31u;  // rank tag: particle left the 27-brick neighbourhood
 
-// Window of one chunk: origin and extent in cells; wx == 0 marks the
-// fallback (window too large or empty chunk).
-struct Box {
-  int x0, y0, z0, wx, wy, wz;
+// Brick of a cell, clamped into the grid.
+__device__ __forceinline__ void cell_brick(const Params &P, float x, float y, float z, int &bx,
+                                           int &by, int &bz) {
+  bx = min(max((int)floorf(x), 0), (int)P.gx - 1) >> 3;
+  by = min(max((int)floorf(y), 0), (int)P.gy - 1) >> 3;
+  bz = min(max((int)floorf(z), 0), (int)P.gz - 1) >> 3;
+}
+
+__device__ __forceinline__ void brick_coords(const BrickWork &bw, uint32_t b, int &bx, int &by,
+                                             int &bz) {
+  bx = (int)(b % bw.nbx);
+  by = (int)((b / bw.nbx) % bw.nby);
+  bz = (int)(b / (bw.nbx * bw.nby));
+}
+
+__device__ __forceinline__ uint32_t brick_index(const BrickWork &bw, int bx, int by, int bz) {
+  return ((uint32_t)bz * bw.nby + (uint32_t)by) * bw.nbx + (uint32_t)bx;
+}
+
+// Global index of window cell c (edge E, origin o), or -1 outside the grid.
+template <int E>
+__device__ __forceinline__ int window_cell(const Params &P, int ox, int oy, int oz, int c) {
+  int wz = c / (E * E), rem = c - wz * (E * E);
+  int wy = rem / E, wx = rem - wy * E;
+  int x = ox + wx, y = oy + wy, z = oz + wz;
+  if (x < 0 || y < 0 || z < 0 || x >= (int)P.gx || y >= (int)P.gy || z >= (int)P.gz) return -1;
+  return (z * (int)P.gy + y) * (int)P.gx + x;
+}
+
+__device__ __forceinline__ void flag_cell_brick(const BrickWork &bw, int x, int y, int z) {
+  bw.ofl[brick_index(bw, x >> 3, y >> 3, z >> 3)] = 1;
+}
+
+// A particle whose stencil leaves the window core contributes to margin
+// cells (or, past the margin, straight to the overflow grid): flag the
+// bricks of its stencil corners so the resolve folds the overflow in.
+__device__ __forceinline__ void flag_stencil(const Params &P, const BrickWork &bw, const PW &pw) {
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    int x = min(max(pw.b[0] + ((c & 1) ? 2 : 0), 0), (int)P.gx - 1);
+    int y = min(max(pw.b[1] + ((c & 2) ? 2 : 0), 0), (int)P.gy - 1);
+    int z = min(max(pw.b[2] + ((c & 4) ? 2 : 0), 0), (int)P.gz - 1);
+    flag_cell_brick(bw, x, y, z);
+  }
+}
+
+// Window position of a particle's base cell and whether its stencil lies in
+// the window / in the window core.
+struct WinPos {
+  int lb;
+  bool inside, in_core;
 };
-
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-__device__ __forceinline__ uint32_t window_gidx(const Box &bx, const Params &P, uint32_t c) {
-  uint32_t sxy = (uint32_t)(bx.wx * bx.wy);
-  uint32_t cz = c / sxy;
-  uint32_t rem = c - cz * sxy;
-  uint32_t cy = rem / (uint32_t)bx.wx;
-  uint32_t cx = rem - cy * (uint32_t)bx.wx;
-  return ((uint32_t)(bx.z0 + (int)cz) * P.gy + (uint32_t)(bx.y0 + (int)cy)) * P.gx +
-         (uint32_t)(bx.x0 + (int)cx);
-}
-
-// Sort key of the cell holding a particle: Morton rank of its 4x4x4 tile,
-// then x-fastest position inside the tile.
-__device__ __forceinline__ uint32_t cell_key(const Params &P, const KeyOut &ko, float x, float y,
-                                             float z) {
-  int cx = min(max((int)floorf(x), 0), (int)P.gx - 1);
-  int cy = min(max((int)floorf(y), 0), (int)P.gy - 1);
-  int cz = min(max((int)floorf(z), 0), (int)P.gz - 1);
-  uint32_t tile = ((uint32_t)(cz >> 2) * ko.tdy + (uint32_t)(cy >> 2)) * ko.tdx + (uint32_t)(cx >> 2);
-  return ko.tile_rank[tile] * 64u + (uint32_t)(((cz & 3) << 4) | ((cy & 3) << 2) | (cx & 3));
-}
-__device__ __forceinline__ void record_key(const Params &P, const KeyOut &ko, uint32_t s, float x,
-                                           float y, float z) {
-  uint32_t key = cell_key(P, ko, x, y, z);
-  ko.keys[s] = key;
-  ko.rank[s] = atomicAdd(&ko.counts[key], 1u);
+template <int M>
+__device__ __forceinline__ WinPos win_pos(const PW &pw, int ox, int oy, int oz) {
+  constexpr int E = 10 + 2 * M;
+  int lx = pw.b[0] - ox, ly = pw.b[1] - oy, lz = pw.b[2] - oz;
+  WinPos w;
+  w.inside = lx >= 0 && ly >= 0 && lz >= 0 && lx + 2 < E && ly + 2 < E && lz + 2 < E;
+  w.in_core = lx >= M && ly >= M && lz >= M && lx + 2 < M + kCore && ly + 2 < M + kCore &&
+              lz + 2 < M + kCore;
+  w.lb = (lz * E + ly) * E + lx;
+  return w;
 }
 
 // ---------------------------------------------------------------------------
-// P2G mass, tiled.  Also writes the chunk's window for the momentum pass and,
-// when ko.counts is set, the cell key and rank of every particle for the
-// re-layout done by G2P.
-template <int SL, bool UNIT>
-__global__ __launch_bounds__(kTB) void k_p2g_mass_tiled(Params P, PState S, GState g,
-                                                        int32_t *boxes, uint32_t wmax,
+// P2G mass over brick work items (WGSL_P2G_MASS, shaders.js:242-274).  With
+// ko.counts set, also assigns every particle its brick and rank for the
+// re-sort.
+template <int M, bool UNIT>
+__global__ __launch_bounds__(256) void k_p2g_mass_brick(Params P, PState S, BrickWork bw,
                                                         KeyOut ko) {
-  constexpr int PER = SL / 4;  // particles per thread
-  constexpr int CH = 64 * SL;
-  extern __shared__ int32_t lds[];
-  __shared__ int red[6][kTB / 64];
-  __shared__ Box sbox;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const uint32_t cbase = blockIdx.x * (uint32_t)CH;
-
-  float px[PER], py[PER], pz[PER];
-  int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
-#pragma unroll
-  for (int r = 0; r < PER; r++) {
-    uint32_t s = cbase + (uint32_t)((wv * PER + r) * 64 + lane);
-    if (s < P.n) {
-      px[r] = S.at(PX, s);
-      py[r] = S.at(PY, s);
-      pz[r] = S.at(PZ, s);
-      int bx = (int)floorf(px[r]) - 1, by = (int)floorf(py[r]) - 1, bz = (int)floorf(pz[r]) - 1;
-      if (ko.counts) record_key(P, ko, s, px[r], py[r], pz[r]);
-      mn[0] = min(mn[0], bx); mx[0] = max(mx[0], bx);
-      mn[1] = min(mn[1], by); mx[1] = max(mx[1], by);
-      mn[2] = min(mn[2], bz); mx[2] = max(mx[2], bz);
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    int lo = wave_min(mn[a]), hi = wave_max(mx[a]);
-    if (lane == 0) {
-      red[a][wv] = lo;
-      red[3 + a][wv] = hi;
-    }
-  }
+  constexpr int E = 10 + 2 * M, V = E * E * E;
+  __shared__ int32_t lds[V];
+  __shared__ uint32_t lcnt[27], lbase[27];
+  if (blockIdx.x >= *bw.nitems) return;
+  const uint4 it = bw.items[blockIdx.x];
+  int ibx, iby, ibz;
+  brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
+  const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int c = t; c < V; c += nt) lds[c] = 0;
+  if (t < 27) lcnt[t] = 0;
   __syncthreads();
-  if (t == 0) {
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; a++) {
-      lo[a] = red[a][0];
-      hi[a] = red[3 + a][0];
-      for (int k = 1; k < kTB / 64; k++) {
-        lo[a] = min(lo[a], red[a][k]);
-        hi[a] = max(hi[a], red[3 + a][k]);
+  for (uint32_t idx = t; idx < it.z; idx += nt) {
+    uint32_t s = it.y + idx;
+    float px = S.at(PX, s), py = S.at(PY, s), pz = S.at(PZ, s);
+    PW pw;
+    particle_weights(px, py, pz, pw);
+    if (ko.counts) {
+      int bx, by, bz;
+      cell_brick(P, px, py, pz, bx, by, bz);
+      int dx = bx - ibx + 1, dy = by - iby + 1, dz = bz - ibz + 1;
+      uint32_t dest = bw.brick_rank[brick_index(bw, bx, by, bz)];
+      ko.keys[s] = dest;
+      if (dx >= 0 && dx < 3 && dy >= 0 && dy < 3 && dz >= 0 && dz < 3) {
+        uint32_t nb = (uint32_t)((dz * 3 + dy) * 3 + dx);
+        ko.rank[s] = (nb << 26) | atomicAdd(&lcnt[nb], 1u);
+      } else {
+        ko.rank[s] = (kFar << 26) | atomicAdd(&ko.counts[dest], 1u);
       }
     }
-    Box b{0, 0, 0, 0, 0, 0};
-    if (lo[0] <= hi[0]) {
-      int wx = hi[0] - lo[0] + 3, wy = hi[1] - lo[1] + 3, wz = hi[2] - lo[2] + 3;
-      if ((uint64_t)wx * (uint64_t)wy * (uint64_t)wz <= (uint64_t)wmax)
-        b = Box{lo[0], lo[1], lo[2], wx, wy, wz};
-    }
-    sbox = b;
-    int32_t *o = boxes + 6 * blockIdx.x;
-    o[0] = b.x0; o[1] = b.y0; o[2] = b.z0; o[3] = b.wx; o[4] = b.wy; o[5] = b.wz;
-  }
-  __syncthreads();
-  const Box bx = sbox;
-
-  if (bx.wx == 0) {  // fallback: direct global atomics (WGSL_P2G_MASS as written)
+    float pm = UNIT ? 1.0f : S.at(PM, s);
+    WinPos wp = win_pos<M>(pw, ox, oy, oz);
+    if (!wp.in_core) flag_stencil(P, bw, pw);
+    if (wp.inside) {
 #pragma unroll
-    for (int r = 0; r < PER; r++) {
-      uint32_t s = cbase + (uint32_t)((wv * PER + r) * 64 + lane);
-      if (s >= P.n) continue;
-      PW pw;
-      particle_weights(px[r], py[r], pz[r], pw);
-      float pm = UNIT ? 1.0f : S.at(PM, s);
-      for (int k = 0; k < 3; k++)
+      for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          float wxy[3];
+#pragma unroll
+          for (int i = 0; i < 3; i++) wxy[i] = pw.w[0][i] * pw.w[1][j];
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            float w = wxy[i] * pw.w[2][kk];
+            atomicAdd(&lds[wp.lb + (kk * E + j) * E + i], encode(UNIT ? w : w * pm));
+          }
+        }
+    } else {  // beyond the window: overflow atomics
+      for (int kk = 0; kk < 3; kk++)
         for (int j = 0; j < 3; j++)
           for (int i = 0; i < 3; i++) {
-            int idx = cell_idx_checked(pw.b[0] + i, pw.b[1] + j, pw.b[2] + k, P);
-            if (idx < 0) continue;
-            float w = (pw.w[0][i] * pw.w[1][j]) * pw.w[2][k];
-            atomicAdd(&g.m[idx], encode(UNIT ? w : w * pm));
+            int gi = cell_idx_checked(pw.b[0] + i, pw.b[1] + j, pw.b[2] + kk, P);
+            if (gi < 0) continue;
+            float w = (pw.w[0][i] * pw.w[1][j]) * pw.w[2][kk];
+            atomicAdd(&bw.om[gi], encode(UNIT ? w : w * pm));
           }
     }
-    return;
   }
-
-  const uint32_t vol = (uint32_t)(bx.wx * bx.wy * bx.wz);
-  for (uint32_t c = t; c < vol; c += kTB) lds[c] = 0;
   __syncthreads();
-  const int sxy = bx.wx * bx.wy;
-#pragma unroll
-  for (int r = 0; r < PER; r++) {
-    uint32_t s = cbase + (uint32_t)((wv * PER + r) * 64 + lane);
-    if (s >= P.n) continue;
-    PW pw;
-    particle_weights(px[r], py[r], pz[r], pw);
-    float pm = UNIT ? 1.0f : S.at(PM, s);
-    int lb = ((pw.b[2] - bx.z0) * bx.wy + (pw.b[1] - bx.y0)) * bx.wx + (pw.b[0] - bx.x0);
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        int row = lb + k * sxy + j * bx.wx;
-        float wxy[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) wxy[i] = pw.w[0][i] * pw.w[1][j];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          float w = wxy[i] * pw.w[2][k];
-          atomicAdd(&lds[row + i], encode(UNIT ? w : w * pm));
-        }
+  if (ko.counts) {
+    if (t < 27) {
+      uint32_t c = lcnt[t], base = 0;
+      if (c) {
+        int bx = ibx + t % 3 - 1, by = iby + (t / 3) % 3 - 1, bz = ibz + t / 9 - 1;
+        base = atomicAdd(&ko.counts[bw.brick_rank[brick_index(bw, bx, by, bz)]], c);
       }
+      lbase[t] = base;
+    }
+    __syncthreads();
+    for (uint32_t idx = t; idx < it.z; idx += nt) {
+      uint32_t s = it.y + idx;
+      uint32_t r = ko.rank[s], nb = r >> 26;
+      if (nb != kFar) ko.rank[s] = lbase[nb] + (r & 0x3ffffffu);
+      else ko.rank[s] = r & 0x3ffffffu;
+    }
   }
-  __syncthreads();
-  for (uint32_t c = t; c < vol; c += kTB) {
-    int32_t v = lds[c];
-    if (v != 0) atomicAdd(&g.m[window_gidx(bx, P, c)], v);
+  // core to the ghost buffer, non-zero margin cells to the overflow grid
+  int32_t *gh = bw.gm + (size_t)blockIdx.x * kCoreV;
+  for (int c = t; c < kCoreV; c += nt) {
+    int cz = c / 100, cy = (c / 10) % 10, cx = c % 10;
+    gh[c] = lds[((cz + M) * E + (cy + M)) * E + (cx + M)];
+  }
+  if (M > 0) {
+    for (int c = t; c < V; c += nt) {
+      int32_t v = lds[c];
+      if (v == 0) continue;
+      int wz = c / (E * E), wy = (c / E) % E, wx = c % E;
+      bool core = wx >= M && wy >= M && wz >= M && wx < M + kCore && wy < M + kCore &&
+                  wz < M + kCore;
+      if (core) continue;
+      int gi = window_cell<E>(P, ox, oy, oz, c);
+      if (gi >= 0) atomicAdd(&bw.om[gi], v);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// P2G momentum, tiled: the decoded mass window and the momentum window live
-// in LDS; density is gathered from LDS.
-template <int SL, bool UNIT>
-__global__ __launch_bounds__(kTB) void k_p2g_mom_tiled(Params P, PState S, GState g,
-                                                       const int32_t *boxes, uint32_t wmax) {
-  constexpr int PER = SL / 4;
-  constexpr int CH = 64 * SL;
-  extern __shared__ int32_t lds[];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const uint32_t cbase = blockIdx.x * (uint32_t)CH;
-  const int32_t *bp = boxes + 6 * blockIdx.x;
-  const Box bx{bp[0], bp[1], bp[2], bp[3], bp[4], bp[5]};
-
-  if (bx.wx == 0) {  // fallback: decode on the fly, global atomics
-    const uint32_t gx = P.gx, slice = P.gx * P.gy;
-    for (int r = 0; r < PER; r++) {
-      uint32_t s = cbase + (uint32_t)((wv * PER + r) * 64 + lane);
-      if (s >= P.n) continue;
-      PW pw;
-      particle_weights(S.at(PX, s), S.at(PY, s), S.at(PZ, s), pw);
-      uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
-      p2g_mom_particle<UNIT>(
-          P, S, s, pw,
-          [&](int k, int j, int i) { return decode(g.m[base + k * slice + j * gx + i]); },
-          [&](int k, int j, int i, int32_t ex, int32_t ey, int32_t ez) {
-            uint32_t c = base + k * slice + j * gx + i;
-            atomicAdd(&g.mx[c], ex);
-            atomicAdd(&g.my[c], ey);
-            atomicAdd(&g.mz[c], ez);
-          });
-    }
-    return;
-  }
-
-  float *lm = (float *)lds;      // decoded mass, wmax
-  int32_t *lv = lds + wmax;      // momentum, 3 interleaved per cell
-  const uint32_t vol = (uint32_t)(bx.wx * bx.wy * bx.wz);
-  for (uint32_t c = t; c < vol; c += kTB) {
-    lm[c] = decode(g.m[window_gidx(bx, P, c)]);
+// P2G momentum over brick work items (WGSL_P2G_MOM, shaders.js:286-370): the
+// decoded mass window and the momentum window live in LDS.
+template <int M, bool UNIT>
+__global__ __launch_bounds__(256) void k_p2g_mom_brick(Params P, PState S, GState g,
+                                                       BrickWork bw) {
+  constexpr int E = 10 + 2 * M, V = E * E * E;
+  __shared__ float lm[V];
+  __shared__ int32_t lv[3 * V];
+  if (blockIdx.x >= *bw.nitems) return;
+  const uint4 it = bw.items[blockIdx.x];
+  int ibx, iby, ibz;
+  brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
+  const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int c = t; c < V; c += nt) {
+    int gi = window_cell<E>(P, ox, oy, oz, c);
+    lm[c] = gi >= 0 ? decode(g.m[gi]) : 0.0f;
     lv[3 * c] = 0;
     lv[3 * c + 1] = 0;
     lv[3 * c + 2] = 0;
   }
   __syncthreads();
-  const int sxy = bx.wx * bx.wy;
-  for (int r = 0; r < PER; r++) {
-    uint32_t s = cbase + (uint32_t)((wv * PER + r) * 64 + lane);
-    if (s >= P.n) continue;
+  const uint32_t gx = P.gx, slice = P.gx * P.gy;
+  // Software pipeline: the next particle's inputs load while this one computes.
+  uint32_t idx = t;
+  MomIn cur;
+  if (idx < it.z) load_mom_in<UNIT>(S, it.y + idx, cur);
+  while (idx < it.z) {
+    const uint32_t s = it.y + idx, nidx = idx + nt;
+    MomIn nxt;
+    if (nidx < it.z) load_mom_in<UNIT>(S, it.y + nidx, nxt);
     PW pw;
-    particle_weights(S.at(PX, s), S.at(PY, s), S.at(PZ, s), pw);
-    int lb = ((pw.b[2] - bx.z0) * bx.wy + (pw.b[1] - bx.y0)) * bx.wx + (pw.b[0] - bx.x0);
-    p2g_mom_particle<UNIT>(
-        P, S, s, pw, [&](int k, int j, int i) { return lm[lb + k * sxy + j * bx.wx + i]; },
-        [&](int k, int j, int i, int32_t ex, int32_t ey, int32_t ez) {
-          int c = lb + k * sxy + j * bx.wx + i;
-          atomicAdd(&lv[3 * c], ex);
-          atomicAdd(&lv[3 * c + 1], ey);
-          atomicAdd(&lv[3 * c + 2], ez);
-        });
+    particle_weights(cur.px, cur.py, cur.pz, pw);
+    WinPos wp = win_pos<M>(pw, ox, oy, oz);
+    float density;
+    if (wp.inside) {
+      const int lb = wp.lb;
+      density = p2g_mom_body<UNIT>(
+          P, cur, pw, [&](int kk, int j, int i) { return lm[lb + (kk * E + j) * E + i]; },
+          [&](int kk, int j, int i, int32_t ex, int32_t ey, int32_t ez) {
+            int c = lb + (kk * E + j) * E + i;
+            atomicAdd(&lv[3 * c], ex);
+            atomicAdd(&lv[3 * c + 1], ey);
+            atomicAdd(&lv[3 * c + 2], ez);
+          });
+    } else {  // beyond the window: global gather, overflow atomics
+      uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
+      density = p2g_mom_body<UNIT>(
+          P, cur, pw,
+          [&](int kk, int j, int i) { return decode(g.m[base + kk * slice + j * gx + i]); },
+          [&](int kk, int j, int i, int32_t ex, int32_t ey, int32_t ez) {
+            uint32_t c = base + kk * slice + j * gx + i;
+            atomicAdd(&bw.ox[c], ex);
+            atomicAdd(&bw.oy[c], ey);
+            atomicAdd(&bw.oz[c], ez);
+          });
+    }
+    S.density[s] = density;
+    cur = nxt;
+    idx = nidx;
   }
   __syncthreads();
-  for (uint32_t c = t; c < vol; c += kTB) {
-    int32_t ex = lv[3 * c], ey = lv[3 * c + 1], ez = lv[3 * c + 2];
-    if ((ex | ey | ez) == 0) continue;
-    uint32_t gi = window_gidx(bx, P, c);
-    if (ex) atomicAdd(&g.mx[gi], ex);
-    if (ey) atomicAdd(&g.my[gi], ey);
-    if (ez) atomicAdd(&g.mz[gi], ez);
+  const size_t gb = (size_t)blockIdx.x * kCoreV;
+  for (int c = t; c < kCoreV; c += nt) {
+    int cz = c / 100, cy = (c / 10) % 10, cx = c % 10;
+    int w = ((cz + M) * E + (cy + M)) * E + (cx + M);
+    bw.gx[gb + c] = lv[3 * w];
+    bw.gy[gb + c] = lv[3 * w + 1];
+    bw.gz[gb + c] = lv[3 * w + 2];
+  }
+  if (M > 0) {
+    for (int c = t; c < V; c += nt) {
+      int32_t ex = lv[3 * c], ey = lv[3 * c + 1], ez = lv[3 * c + 2];
+      if ((ex | ey | ez) == 0) continue;
+      int wz = c / (E * E), wy = (c / E) % E, wx = c % E;
+      bool core = wx >= M && wy >= M && wz >= M && wx < M + kCore && wy < M + kCore &&
+                  wz < M + kCore;
+      if (core) continue;
+      int gi = window_cell<E>(P, ox, oy, oz, c);
+      if (gi < 0) continue;
+      if (ex) atomicAdd(&bw.ox[gi], ex);
+      if (ey) atomicAdd(&bw.oy[gi], ey);
+      if (ez) atomicAdd(&bw.oz[gi], ez);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Counting sort by cell key.
-__global__ __launch_bounds__(256) void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = v;
+// Resolve.  list holds the bricks whose cells can be non-zero this step or
+// were non-zero last step: bricks next to an occupied brick, flagged bricks
+// and bricks active last step.  One block (512 threads, one per cell) per
+// listed brick, persistent over the list.
+__global__ __launch_bounds__(256) void k_brick_list(BrickWork bw) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= bw.nbricks) return;
+  int bx, by, bz;
+  brick_coords(bw, b, bx, by, bz);
+  bool need = (bw.act[b] & 1) || bw.ofl[b];
+  for (int n = 0; n < 27 && !need; n++) {
+    int x = bx + n % 3 - 1, y = by + (n / 3) % 3 - 1, z = bz + n / 9 - 1;
+    if (x < 0 || y < 0 || z < 0 || x >= (int)bw.nbx || y >= (int)bw.nby || z >= (int)bw.nbz)
+      continue;
+    need = bw.occ[brick_index(bw, x, y, z)] != 0;
+  }
+  if (need) bw.list[atomicAdd(bw.nlist, 1u)] = b;
 }
 
-__global__ __launch_bounds__(256) void k_sort_keys(Params P, PState S, KeyOut ko) {
+struct NbrItems {
+  uint32_t first[27], count[27];
+};
+
+__device__ __forceinline__ bool load_neighbours(const BrickWork &bw, uint32_t b, NbrItems &nb,
+                                                int t) {
+  __shared__ int any;
+  if (t == 0) any = 0;
+  __syncthreads();
+  if (t < 27) {
+    int bx, by, bz;
+    brick_coords(bw, b, bx, by, bz);
+    bx += t % 3 - 1;
+    by += (t / 3) % 3 - 1;
+    bz += t / 9 - 1;
+    uint32_t cnt = 0, first = 0;
+    if (bx >= 0 && by >= 0 && bz >= 0 && bx < (int)bw.nbx && by < (int)bw.nby &&
+        bz < (int)bw.nbz) {
+      uint32_t r = bw.brick_rank[brick_index(bw, bx, by, bz)];
+      cnt = bw.count[r];
+      first = bw.first[r];
+    }
+    nb.first[t] = first;
+    nb.count[t] = cnt;
+    if (cnt) atomicOr(&any, 1);
+  }
+  __syncthreads();
+  return any != 0;
+}
+
+// Calls f(ghost index) for every ghost core that covers cell (lx,ly,lz) of
+// the brick (local coords 0..7): its own brick's items, and the neighbour
+// bricks' items whose core reaches one cell into this brick.
+template <class F>
+__device__ __forceinline__ void for_covering_items(const NbrItems &nb, int lx, int ly, int lz,
+                                                   F f) {
+  const int x0 = lx == 0 ? 0 : 1, x1 = lx == 7 ? 2 : 1;
+  const int y0 = ly == 0 ? 0 : 1, y1 = ly == 7 ? 2 : 1;
+  const int z0 = lz == 0 ? 0 : 1, z1 = lz == 7 ? 2 : 1;
+  for (int dz = z0; dz <= z1; dz++)
+    for (int dy = y0; dy <= y1; dy++)
+      for (int dx = x0; dx <= x1; dx++) {
+        int n = (dz * 3 + dy) * 3 + dx;
+        if (!nb.count[n]) continue;
+        // core coordinates of this cell in brick n (core origin = brick n - 1)
+        int cx = lx - (dx - 1) * 8 + 1, cy = ly - (dy - 1) * 8 + 1, cz = lz - (dz - 1) * 8 + 1;
+        int li = (cz * kCore + cy) * kCore + cx;
+        for (uint32_t k = 0; k < nb.count[n]; k++) f((size_t)(nb.first[n] + k) * kCoreV + li);
+      }
+}
+
+__global__ __launch_bounds__(512) void k_resolve_mass(Params P, GState g, BrickWork bw) {
+  __shared__ NbrItems nb;
+  const int t = threadIdx.x;
+  const uint32_t nl = *bw.nlist;
+  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) {
+    const uint32_t b = bw.list[i];
+    bool any = load_neighbours(bw, b, nb, t);
+    const bool ofl = bw.ofl[b] != 0;
+    int bx, by, bz;
+    brick_coords(bw, b, bx, by, bz);
+    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+    const int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
+    if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
+      uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
+      uint32_t sum = 0;
+      if (any) for_covering_items(nb, lx, ly, lz, [&](size_t gi) { sum += (uint32_t)bw.gm[gi]; });
+      if (ofl) {
+        sum += (uint32_t)bw.om[cell];
+        bw.om[cell] = 0;
+      }
+      g.m[cell] = (int32_t)sum;
+    }
+    if (t == 0) {
+      uint8_t a = bw.act[b];
+      bw.act[b] = (uint8_t)(((a & 1) << 1) | ((any || ofl) ? 1 : 0));
+    }
+    __syncthreads();
+  }
+}
+
+// Momentum resolve fused with the grid update (WGSL_UPDATE_GRID,
+// shaders.js:443-512): (vx, vy, vz, decoded mass) per cell.
+__global__ __launch_bounds__(512) void k_resolve_update(Params P, GState g, BrickWork bw, Walls W,
+                                                        int keep) {
+  __shared__ NbrItems nb;
+  const int t = threadIdx.x;
+  const uint32_t nl = *bw.nlist;
+  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) {
+    const uint32_t b = bw.list[i];
+    bool any = load_neighbours(bw, b, nb, t);
+    const bool ofl = bw.ofl[b] != 0;
+    int bx, by, bz;
+    brick_coords(bw, b, bx, by, bz);
+    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+    const int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
+    if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
+      uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
+      uint32_t sx = 0, sy = 0, sz = 0;
+      if (any)
+        for_covering_items(nb, lx, ly, lz, [&](size_t gi) {
+          sx += (uint32_t)bw.gx[gi];
+          sy += (uint32_t)bw.gy[gi];
+          sz += (uint32_t)bw.gz[gi];
+        });
+      if (ofl) {
+        sx += (uint32_t)bw.ox[cell];
+        sy += (uint32_t)bw.oy[cell];
+        sz += (uint32_t)bw.oz[cell];
+        bw.ox[cell] = 0;
+        bw.oy[cell] = 0;
+        bw.oz[cell] = 0;
+      }
+      g.v[cell] = update_cell(P, W, g.m[cell], (int32_t)sx, (int32_t)sy, (int32_t)sz, x, y, z);
+      if (keep) {
+        g.mx[cell] = (int32_t)sx;
+        g.my[cell] = (int32_t)sy;
+        g.mz[cell] = (int32_t)sz;
+      }
+    }
+    if (t == 0) bw.ofl[b] = 0;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// G2P over brick work items (WGSL_G2P, shaders.js:589-704).  With src_of set,
+// slot d of the destination state takes the particle at old slot src_of[d]
+// (re-sort; mass, id and density move along) and every particle lies in the
+// item's brick (EXACT: no particle can leave the window).  Otherwise G2P runs
+// in place over the current items with the drift margin.
+template <int M, bool EXACT>
+__global__ __launch_bounds__(256) void k_g2p_brick(Params P, PState S, PState D, GState g,
+                                                   Walls W, BrickWork bw,
+                                                   const uint32_t *src_of) {
+  constexpr int E = 10 + 2 * M, V = E * E * E;
+  __shared__ float4 lv[V];
+  if (blockIdx.x >= *bw.nitems) return;
+  const uint4 it = bw.items[blockIdx.x];
+  int ibx, iby, ibz;
+  brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
+  const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int c = t; c < V; c += nt) {
+    int gi = window_cell<E>(P, ox, oy, oz, c);
+    lv[c] = gi >= 0 ? g.v[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  __syncthreads();
+  const uint32_t gx = P.gx, slice = P.gx * P.gy;
+  // Software pipeline: the next particle's source slot and inputs load while
+  // this one computes.
+  struct In {
+    uint32_t s;
+    float px, py, pz;
+  };
+  auto load_in = [&](uint32_t d, In &in) {
+    in.s = EXACT ? src_of[d] : d;
+    in.px = S.at(PX, in.s);
+    in.py = S.at(PY, in.s);
+    in.pz = S.at(PZ, in.s);
+  };
+  uint32_t idx = t;
+  In cur;
+  if (idx < it.z) load_in(it.y + idx, cur);
+  while (idx < it.z) {
+    const uint32_t d = it.y + idx, nidx = idx + nt;
+    In nxt;
+    if (nidx < it.z) load_in(it.y + nidx, nxt);
+    const uint32_t s = cur.s;
+    const float px = cur.px, py = cur.py, pz = cur.pz;
+    PW pw;
+    particle_weights(px, py, pz, pw);
+    WinPos wp = win_pos<M>(pw, ox, oy, oz);
+    G2POut o;
+    if (EXACT || wp.inside) {
+      const int lb = wp.lb;
+      g2p_particle(P, W, px, py, pz, pw, [&](int kk, int j, int i) {
+        float4 v = lv[lb + (kk * E + j) * E + i];
+        return make_float3(v.x, v.y, v.z);
+      }, o);
+    } else {
+      uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
+      g2p_particle(P, W, px, py, pz, pw, [&](int kk, int j, int i) {
+        float4 v = g.v[base + kk * slice + j * gx + i];
+        return make_float3(v.x, v.y, v.z);
+      }, o);
+    }
+    store_g2p(D, d, o);
+    if (EXACT) {
+      D.at(PM, d) = S.at(PM, s);
+      D.id[d] = S.id[s];
+      D.density[d] = S.density[s];
+    }
+    cur = nxt;
+    idx = nidx;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sort plumbing.
+// First-step keys: brick and rank of every particle by global atomics.
+__global__ __launch_bounds__(256) void k_sort_keys(Params P, PState S, BrickWork bw, KeyOut ko) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
-  record_key(P, ko, s, S.at(PX, s), S.at(PY, s), S.at(PZ, s));
+  int bx, by, bz;
+  cell_brick(P, S.at(PX, s), S.at(PY, s), S.at(PZ, s), bx, by, bz);
+  uint32_t dest = bw.brick_rank[brick_index(bw, bx, by, bz)];
+  ko.keys[s] = dest;
+  ko.rank[s] = atomicAdd(&ko.counts[dest], 1u);
 }
 
-// Slot of sorted rank q: inside a full chunk of 64*sl particles, rank
-// lane*sl + slice goes to slice*64 + lane; the last partial chunk keeps order.
-__device__ __forceinline__ uint32_t slot_of_rank(uint32_t q, uint32_t n, uint32_t sl) {
-  const uint32_t ch = 64u * sl;
-  uint32_t full = (n / ch) * ch;
-  if (q >= full) return q;
-  uint32_t r = q % ch;
-  return (q - r) + (r % sl) * 64u + r / sl;
+// Work items from the brick offsets: each brick's particle range cut into
+// pieces of at most `ch` particles.  Item order is irrelevant to the result.
+__global__ __launch_bounds__(256) void k_make_items(uint32_t n, BrickWork bw, uint32_t ch,
+                                                    const uint32_t *offsets) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= bw.nbricks) return;
+  uint32_t start = offsets[r];
+  uint32_t end = (r + 1 < bw.nbricks) ? offsets[r + 1] : n;
+  uint32_t b = bw.brick_of_rank[r];
+  if (end <= start) {
+    bw.count[r] = 0;
+    bw.first[r] = 0;
+    bw.occ[b] = 0;
+    return;
+  }
+  uint32_t cnt = end - start;
+  uint32_t pieces = (cnt + ch - 1) / ch;
+  uint32_t at = atomicAdd(bw.nitems, pieces);
+  bw.first[r] = at;
+  bw.count[r] = pieces;
+  bw.occ[b] = 1;
+  for (uint32_t p = 0; p < pieces; p++) {
+    uint32_t c = cnt - p * ch;
+    bw.items_w[at + p] = make_uint4(r, start + p * ch, c < ch ? c : ch, 0u);
+  }
 }
 
-// Inverse permutation for the G2P re-layout: src_of[new slot] = old slot.
-__global__ __launch_bounds__(256) void k_build_src(uint32_t n, uint32_t sl, const uint32_t *keys,
-                                                   const uint32_t *rank, const uint32_t *offsets,
-                                                   uint32_t *src_of) {
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+
+// Slot of the particle of rank r among the c particles of an item: ranks are
+// stored column-major over rows of 256 slots.  With K = ceil(c/256), the
+// first L = c - 256(K-1) columns hold K ranks, the others K-1.
+__device__ __forceinline__ uint32_t item_slot(uint32_t r, uint32_t c) {
+  uint32_t K = (c + 255u) / 256u;
+  uint32_t L = c - 256u * (K - 1u);
+  uint32_t t, k;
+  if (r < L * K) {
+    t = r / K;
+    k = r - t * K;
+  } else {
+    uint32_t r2 = r - L * K;
+    t = L + r2 / (K - 1u);
+    k = r2 % (K - 1u);
+  }
+  return k * 256u + t;
+}
+
+// Inverse permutation of the re-sort: src_of[new slot] = old slot.
+__global__ __launch_bounds__(256) void k_build_src(uint32_t n, uint32_t nbricks, uint32_t ch,
+                                                   const uint32_t *keys, const uint32_t *rank,
+                                                   const uint32_t *offsets, uint32_t *src_of) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  src_of[slot_of_rank(offsets[keys[s]] + rank[s], n, sl)] = s;
+  uint32_t brick = keys[s];
+  uint32_t bstart = offsets[brick];
+  uint32_t bend = (brick + 1 < nbricks) ? offsets[brick + 1] : n;
+  uint32_t local = rank[s];
+  uint32_t piece = local / ch;
+  uint32_t istart = bstart + piece * ch;
+  uint32_t c = bend - istart;
+  if (c > ch) c = ch;
+  src_of[istart + item_slot(local - piece * ch, c)] = s;
 }
 
-__global__ __launch_bounds__(256) void k_sort_scatter(uint32_t n, uint32_t sl, PState src,
-                                                      PState dst, const uint32_t *keys,
-                                                      const uint32_t *rank,
-                                                      const uint32_t *offsets) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  uint32_t d = slot_of_rank(offsets[keys[s]] + rank[s], n, sl);
+// Gather form of the first step's data move: slot d takes old slot src_of[d].
+__global__ __launch_bounds__(256) void k_sort_gather(uint32_t n, PState src, PState dst,
+                                                     const uint32_t *src_of) {
+  uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= n) return;
+  uint32_t s = src_of[d];
 #pragma unroll
   for (int k = 0; k < NPLANES; k++) dst.at(k, d) = src.at(k, s);
   dst.id[d] = src.id[s];
@@ -312,77 +607,98 @@ __global__ __launch_bounds__(256) void k_sort_scatter(uint32_t n, uint32_t sl, P
 // Host launchers.
 static inline uint32_t nblk(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
-template <int SL>
-static void mass_tiled(const Params &P, const PState &S, const GState &g, int32_t *boxes,
-                       uint32_t wmax, const KeyOut &ko, hipStream_t st) {
-  uint32_t nb = nblk(P.n, 64u * SL);
-  size_t shm = (size_t)wmax * 4;
-  if (P.unit_mass) k_p2g_mass_tiled<SL, true><<<nb, kTB, shm, st>>>(P, S, g, boxes, wmax, ko);
-  else k_p2g_mass_tiled<SL, false><<<nb, kTB, shm, st>>>(P, S, g, boxes, wmax, ko);
-}
-template <int SL>
-static void mom_tiled(const Params &P, const PState &S, const GState &g, const int32_t *boxes,
-                      uint32_t wmax, hipStream_t st) {
-  uint32_t nb = nblk(P.n, 64u * SL);
-  size_t shm = (size_t)wmax * 16;
-  if (P.unit_mass) k_p2g_mom_tiled<SL, true><<<nb, kTB, shm, st>>>(P, S, g, boxes, wmax);
-  else k_p2g_mom_tiled<SL, false><<<nb, kTB, shm, st>>>(P, S, g, boxes, wmax);
+bool tiled_ppt_supported(uint32_t ppt) { return ppt == 4 || ppt == 8 || ppt == 16; }
+bool tiled_margin_supported(uint32_t m) { return m == 1 || m == 2; }
+bool tiled_threads_supported(uint32_t t) { return t == 256; }
+
+uint32_t max_items(uint32_t n, uint32_t nbricks, uint32_t ppt) {
+  return nbricks + n / (kTB * ppt) + 1;
 }
 
-bool tiled_slices_supported(uint32_t sl) { return sl == 4 || sl == 8 || sl == 16 || sl == 32; }
+static uint32_t resolve_grid(const BrickWork &bw) { return bw.nbricks < 2048 ? bw.nbricks : 2048; }
 
 void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
-                           int32_t *boxes, const KeyOut &ko, hipStream_t st) {
+                           const BrickWork &bw, const KeyOut &ko, hipStream_t st) {
   if (!P.n) return;
-  switch (tc.slices) {
-    case 4: mass_tiled<4>(P, S, g, boxes, tc.wmax, ko, st); break;
-    case 8: mass_tiled<8>(P, S, g, boxes, tc.wmax, ko, st); break;
-    case 16: mass_tiled<16>(P, S, g, boxes, tc.wmax, ko, st); break;
-    default: mass_tiled<32>(P, S, g, boxes, tc.wmax, ko, st); break;
+  uint32_t grid = max_items(P.n, bw.nbricks, tc.ppt), tb = tc.threads;
+  if (tc.margin == 1) {
+    if (P.unit_mass) k_p2g_mass_brick<1, true><<<grid, tb, 0, st>>>(P, S, bw, ko);
+    else k_p2g_mass_brick<1, false><<<grid, tb, 0, st>>>(P, S, bw, ko);
+  } else {
+    if (P.unit_mass) k_p2g_mass_brick<2, true><<<grid, tb, 0, st>>>(P, S, bw, ko);
+    else k_p2g_mass_brick<2, false><<<grid, tb, 0, st>>>(P, S, bw, ko);
   }
+  k_zero_u32<<<1, 1, 0, st>>>(bw.nlist);
+  k_brick_list<<<nblk(bw.nbricks, 256), 256, 0, st>>>(bw);
+  k_resolve_mass<<<resolve_grid(bw), 512, 0, st>>>(P, g, bw);
 }
 
 void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
-                          const int32_t *boxes, hipStream_t st) {
+                          const BrickWork &bw, hipStream_t st) {
   if (!P.n) return;
-  switch (tc.slices) {
-    case 4: mom_tiled<4>(P, S, g, boxes, tc.wmax, st); break;
-    case 8: mom_tiled<8>(P, S, g, boxes, tc.wmax, st); break;
-    case 16: mom_tiled<16>(P, S, g, boxes, tc.wmax, st); break;
-    default: mom_tiled<32>(P, S, g, boxes, tc.wmax, st); break;
+  uint32_t grid = max_items(P.n, bw.nbricks, tc.ppt), tb = tc.threads;
+  if (tc.margin == 1) {
+    if (P.unit_mass) k_p2g_mom_brick<1, true><<<grid, tb, 0, st>>>(P, S, g, bw);
+    else k_p2g_mom_brick<1, false><<<grid, tb, 0, st>>>(P, S, g, bw);
+  } else {
+    if (P.unit_mass) k_p2g_mom_brick<2, true><<<grid, tb, 0, st>>>(P, S, g, bw);
+    else k_p2g_mom_brick<2, false><<<grid, tb, 0, st>>>(P, S, g, bw);
   }
 }
 
-KeyOut key_out(const SortBufs &sb, uint32_t tdx, uint32_t tdy) {
+void launch_resolve_update(const Params &P, const GState &g, const TileCfg &tc,
+                           const BrickWork &bw, const Walls &W, bool keep, hipStream_t st) {
+  (void)tc;
+  k_resolve_update<<<resolve_grid(bw), 512, 0, st>>>(P, g, bw, W, keep ? 1 : 0);
+}
+
+// G2P over the current work items.  Re-sort (src_of set): items of the new
+// order, exact windows.  In place: margin as configured.
+void launch_g2p_tiled(const Params &P, const PState &S, const PState &D, const GState &g,
+                      const Walls &W, const TileCfg &tc, const BrickWork &bw,
+                      const uint32_t *src_of, hipStream_t st) {
+  if (!P.n) return;
+  uint32_t grid = max_items(P.n, bw.nbricks, tc.ppt), tb = tc.g2p_threads;
+  if (src_of) k_g2p_brick<0, true><<<grid, tb, 0, st>>>(P, S, D, g, W, bw, src_of);
+  else if (tc.margin == 1) k_g2p_brick<1, false><<<grid, tb, 0, st>>>(P, S, D, g, W, bw, nullptr);
+  else k_g2p_brick<2, false><<<grid, tb, 0, st>>>(P, S, D, g, W, bw, nullptr);
+}
+
+KeyOut key_out(const SortBufs &sb, const BrickWork &bw) {
+  (void)bw;
   KeyOut ko;
   ko.counts = sb.counts;
   ko.keys = sb.keys;
   ko.rank = sb.rank;
-  ko.tile_rank = sb.tile_rank;
-  ko.tdx = tdx;
-  ko.tdy = tdy;
   return ko;
 }
 
-// Full sort (data moved by a scatter pass); used for the first step after a
-// reset, when the particles are still in caller order.  The histogram array
-// is left zeroed.
-void launch_sort(const Params &P, const PState &src, const PState &dst, const SortBufs &sb,
-                 uint32_t tdx, uint32_t tdy, uint32_t slices, hipStream_t st) {
-  if (!P.n) return;
-  k_sort_keys<<<nblk(P.n, 256), 256, 0, st>>>(P, src, key_out(sb, tdx, tdy));
-  launch_scan(sb.counts, sb.nkeys, false, sb.offsets, sb.scratch, nullptr, st, sb.counts);
-  k_sort_scatter<<<nblk(P.n, 256), 256, 0, st>>>(P.n, slices, src, dst, sb.keys, sb.rank,
-                                                 sb.offsets);
+static void offsets_items_src(const Params &P, const SortBufs &sb, const BrickWork &bw,
+                              const TileCfg &tc, hipStream_t st) {
+  launch_scan(sb.counts, bw.nbricks, false, sb.offsets, sb.scratch, nullptr, st, sb.counts);
+  k_zero_u32<<<1, 1, 0, st>>>(bw.nitems);
+  k_make_items<<<nblk(bw.nbricks, 256), 256, 0, st>>>(P.n, bw, kTB * tc.ppt, sb.offsets);
+  k_build_src<<<nblk(P.n, 256), 256, 0, st>>>(P.n, bw.nbricks, kTB * tc.ppt, sb.keys, sb.rank,
+                                              sb.offsets, sb.src_of);
 }
 
-// After a P2G mass pass that recorded keys: offsets and the inverse
-// permutation for k_g2p_relayout.  Leaves the histogram zeroed.
-void launch_relayout_prep(const Params &P, const SortBufs &sb, uint32_t slices, hipStream_t st) {
+// Full sort for the first step after a reset: brick keys, offsets, work items
+// and a gather of every particle into its slot.  Leaves the counts zeroed.
+void launch_sort(const Params &P, const PState &src, const PState &dst, const SortBufs &sb,
+                 const BrickWork &bw, const TileCfg &tc, hipStream_t st) {
   if (!P.n) return;
-  launch_scan(sb.counts, sb.nkeys, false, sb.offsets, sb.scratch, nullptr, st, sb.counts);
-  k_build_src<<<nblk(P.n, 256), 256, 0, st>>>(P.n, slices, sb.keys, sb.rank, sb.offsets,
-                                              sb.src_of);
+  k_sort_keys<<<nblk(P.n, 256), 256, 0, st>>>(P, src, bw, key_out(sb, bw));
+  offsets_items_src(P, sb, bw, tc, st);
+  k_sort_gather<<<nblk(P.n, 256), 256, 0, st>>>(P.n, src, dst, sb.src_of);
+}
+
+// After a P2G mass pass that recorded keys: offsets, the next step's work
+// items and the inverse permutation for the re-sorting G2P.  Leaves the
+// counts zeroed.
+void launch_relayout_prep(const Params &P, const SortBufs &sb, const BrickWork &bw,
+                          const TileCfg &tc, hipStream_t st) {
+  if (!P.n) return;
+  offsets_items_src(P, sb, bw, tc, st);
 }
 
 }  // namespace mpm

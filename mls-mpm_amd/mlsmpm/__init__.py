@@ -4,11 +4,11 @@ Python side of include/mlsmpm.h: ctypes binding (capi), scene handle
 (scene.Scene), seeded scene layouts (scenes) and the build recipe (build).
 The compute path is libmlsmpm.so only.
 """
-from .capi import (Config, MpmError, MPM_FLAG_DENSE_GRID, MPM_FLAG_KEEP_GRID, MPM_FLAG_NO_LDS,
+from .capi import (Config, MpmError, MPM_FLAG_NO_GRAPH, MPM_FLAG_KEEP_GRID, MPM_FLAG_NO_LDS,
                    config_from_aspect, default_config, load, lib_path)
 from .scene import DREC_DTYPE, Scene
 
 __all__ = [
     "Config", "MpmError", "Scene", "DREC_DTYPE", "default_config", "config_from_aspect", "load",
-    "lib_path", "MPM_FLAG_KEEP_GRID", "MPM_FLAG_NO_LDS", "MPM_FLAG_DENSE_GRID",
+    "lib_path", "MPM_FLAG_KEEP_GRID", "MPM_FLAG_NO_LDS", "MPM_FLAG_NO_GRAPH",
 ]

@@ -23,7 +23,7 @@ MPM_ERR_NOT_FOUND = -8
 
 MPM_FLAG_KEEP_GRID = 1
 MPM_FLAG_NO_LDS = 2
-MPM_FLAG_DENSE_GRID = 4
+MPM_FLAG_NO_GRAPH = 4
 
 ERROR_NAMES = {
     MPM_ERR_INVALID: "MPM_ERR_INVALID", MPM_ERR_RANGE: "MPM_ERR_RANGE",
@@ -55,8 +55,8 @@ class Config(C.Structure):
         ("crest_dot_threshold", C.c_float), ("spawn_rate_k", C.c_float),
         ("lifetime_foam", C.c_float), ("buoyancy", C.c_float), ("drag", C.c_float),
         ("density_spray_max", C.c_float), ("density_bubble_min", C.c_float),
-        ("seed", C.c_uint32), ("tile_slices", C.c_uint32), ("tile_wmax", C.c_uint32),
-        ("reserved", C.c_uint32 * 4),
+        ("seed", C.c_uint32), ("tile_ppt", C.c_uint32), ("tile_margin", C.c_uint32),
+        ("tile_threads", C.c_uint32), ("g2p_threads", C.c_uint32), ("reserved", C.c_uint32 * 2),
     ]
 
     def copy(self) -> "Config":
@@ -91,23 +91,26 @@ _SIGNATURES = {
     "mpm_enable_timing": ([_VP, C.c_int], C.c_int),
     "mpm_stage_times": ([_VP, _VP], C.c_int),
     "mpm_step_times": ([_VP, _VP, C.c_uint32, _VP], C.c_int),
+    "mpm_tile_stats": ([_VP, _VP], C.c_int),
 }
 
 _lib = None
 
 
 def lib_path() -> str:
-    return LIB
+    """The library this process loads: $MLSMPM_LIB if set, else the in-tree build."""
+    return os.environ.get("MLSMPM_LIB") or LIB
 
 
 def load() -> C.CDLL:
     """Load libmlsmpm.so; raises if it has not been built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
-            raise RuntimeError(f"{LIB} is not built: run __graft_entry__.build() "
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built: run __graft_entry__.build() "
                                "or python -m mlsmpm.build (there is no CPU fallback)")
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         for name, (args, res) in _SIGNATURES.items():
             fn = getattr(L, name)
             fn.argtypes = args

@@ -164,6 +164,12 @@ class Scene:
         check(load().mpm_step_times(self._h, _p(ms), cnt.value, C.byref(cnt)))
         return ms[: cnt.value]
 
+    def tile_stats(self) -> dict:
+        o = np.zeros(4, np.uint64)
+        check(load().mpm_tile_stats(self._h, _p(o)))
+        return {"items": int(o[0]), "bricks": int(o[1]), "particles": int(o[2]),
+                "max_item": int(o[3])}
+
     def stream(self) -> int:
         p = C.c_void_p()
         check(load().mpm_get_stream(self._h, C.byref(p)))

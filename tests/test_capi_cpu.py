@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_size_matches_header():
-    # 8 u32 + 22 f32 + seed + 2 tile fields + 4 reserved = 37 words
+    # 8 u32 + 22 f32 + seed + 4 tile fields + 2 reserved = 37 words
     assert C.sizeof(capi.Config) == 37 * 4
 
 
@@ -48,7 +48,7 @@ def test_defaults_are_the_reference_constants():
               "lifetime_foam", "pressure_threshold", "crest_dot_threshold"):
         assert np.float32(getattr(c, f)) == np.float32(getattr(g, f)), f
     assert c.seed == 1 and c.buoyancy == np.float32(1.2) and c.drag == 4.0
-    assert c.sort_period == 8
+    assert c.sort_period == 1
 
 
 def test_config_from_aspect_matches_fluidgpu_constructor():
@@ -67,8 +67,9 @@ def test_config_from_aspect_matches_fluidgpu_constructor():
     ("hard_min", 0.5, capi.MPM_ERR_INVALID),
     ("hard_max_x", 63.5, capi.MPM_ERR_INVALID),
     ("dt", float("nan"), capi.MPM_ERR_INVALID),
-    ("tile_slices", 12, capi.MPM_ERR_INVALID),
-    ("tile_wmax", 8192, capi.MPM_ERR_INVALID),
+    ("tile_ppt", 12, capi.MPM_ERR_INVALID),
+    ("tile_margin", 3, capi.MPM_ERR_INVALID),
+    ("tile_threads", 128, capi.MPM_ERR_INVALID),
 ])
 def test_invalid_configs_are_rejected_before_any_device_work(field, value, code):
     c = mlsmpm.default_config(64, 64, 64, 100)

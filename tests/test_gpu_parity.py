@@ -23,17 +23,17 @@ NO_LDS = mlsmpm.MPM_FLAG_NO_LDS
 #   direct  : per-particle global atomics, particles in reset order
 #   tiled   : LDS windows, particles in reset order
 #   sorted8 : LDS windows, counting sort by cell every 8 steps (the default)
-#   sorted1 : sort every step, 4 slices per lane (256-particle chunks)
-#   sorted32: sort every 3 steps, 32 slices per lane (2048-particle chunks)
-#   spill   : sort every 8 steps with a 64-cell window budget, so most chunks
-#             take the global-atomic fallback
+#   sorted1 : sort every step, 4 particles per thread (1024-particle items)
+#   sorted32: sort every 3 steps, 16 particles per thread, drift margin 2
+#   spill   : sort every 8 steps, margin 1: many particles drift out of their
+#             LDS window and take the overflow-grid path
 MODES = {
     "direct": dict(flags=KEEP | NO_LDS, sort_period=0),
     "tiled": dict(flags=KEEP, sort_period=0),
     "sorted8": dict(flags=KEEP, sort_period=8),
-    "sorted1": dict(flags=KEEP, sort_period=1, tile_slices=4),
-    "sorted32": dict(flags=KEEP, sort_period=3, tile_slices=32),
-    "spill": dict(flags=KEEP, sort_period=8, tile_wmax=64),
+    "sorted1": dict(flags=KEEP, sort_period=1, tile_ppt=4),
+    "sorted32": dict(flags=KEEP, sort_period=3, tile_ppt=16, tile_margin=2),
+    "spill": dict(flags=KEEP, sort_period=8, tile_ppt=8, tile_margin=1),
 }
 
 
@@ -187,3 +187,51 @@ def test_reset_rejects_out_of_grid():
         with pytest.raises(mlsmpm.MpmError) as e:
             s.reset(np.full((11, 3), 5.0, np.float32))
         assert e.value.code == -3
+
+
+NO_GRAPH = mlsmpm.MPM_FLAG_NO_GRAPH
+
+
+@pytest.mark.parametrize("period,timed", [(1, False), (3, False), (8, False), (8, True), (1, True)])
+def test_graph_replay_matches_eager_and_oracle(period, timed):
+    """Multi-step calls replay a captured HIP graph of whole re-sort cycles;
+    the result must equal step-by-step eager enqueue and the CPU restatement,
+    with diffuse particles on, and timing (event nodes in the graph) must
+    report every step."""
+    grid = (40, 32, 36)
+    pos = scenes.lattice_block((2, 2, 2), (22, 22, 20), 0.1, np.random.default_rng(11))
+    D = 20000
+    kw = dict(flags=KEEP, sort_period=period, tile_ppt=4)
+    g, o = make_pair(grid, pos, D=D, **kw)
+    e, _ = make_pair(grid, pos, D=D, **{**kw, "flags": KEEP | NO_GRAPH})
+    if timed:
+        g.enable_timing(True)
+    g.step(1)
+    g.step(50)
+    g.step(19)
+    for _ in range(70):
+        e.step(1)
+    o.step(70)
+    if timed:
+        st = g.step_times()
+        assert len(st) == 70 and np.all(st > 0)
+        assert g.stage_times()[6] > 0
+    check_equal(g, o, "graph vs oracle after 70 steps", diffuse=True)
+    pg, pe = g.read_particles(), e.read_particles()
+    for k in pg:
+        np.testing.assert_array_equal(pg[k], pe[k], err_msg=f"graph vs eager {k}")
+    # parameter change drops the captured graph
+    g.set_params(gravity=-4.0)
+    e.set_params(gravity=-4.0)
+    g.step(40)
+    for _ in range(40):
+        e.step(1)
+    pg, pe = g.read_particles(), e.read_particles()
+    for k in pg:
+        np.testing.assert_array_equal(pg[k], pe[k], err_msg=f"after set_params {k}")
+    rg, flg, tg = g.read_diffuse()
+    re_, fle, te = e.read_diffuse()
+    assert tg == te
+    np.testing.assert_array_equal(flg, fle)
+    g.close()
+    e.close()
