@@ -10,14 +10,18 @@ G2P, plus the periodic sort) over all particles, state resident in HBM.
 
 Prints one JSON line:
   value            particle-steps/s over the timed region (wall clock,
-                   stream-synchronised on both sides)
+                   stream-synchronised on both sides, no events recorded)
   roofline         the dominant kernel's algorithmic bytes per launch over its
-                   mean launch time (HIP events on the scene's stream, every
-                   timed step), against the 8 TB/s HBM peak
+                   mean launch time, from HIP events recorded on the scene's
+                   stream around that kernel alone in a second timed region of
+                   the same length, against the 8 TB/s HBM peak; `traffic` is
+                   the HBM bytes per launch that rocprofv3 FETCH_SIZE +
+                   WRITE_SIZE read for that kernel (profiles/traffic.json),
+                   scaled by the same counters' reading for a plain copy
   cpu_baseline     the CPU restatement of the reference step on this host's
                    cores, on a bounded sample of the same workload
 Extra keys: per-stage mean ms, whole-step roofline fraction, p50/p99 step
-latency.
+latency (from the event-timed region).
 """
 from __future__ import annotations
 
@@ -37,7 +41,8 @@ import numpy as np  # noqa: E402
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 # Algorithmic bytes (information moved, 12 B per vec3) per particle and per
-# active cell for each kernel of the step.
+# active cell for each timed stage of the step.  "g2p", "p2g_mom" and
+# "update" bracket one kernel each (KERNEL names it).
 #   P2G mass     : pos 12 + mass 4 per particle; mass write 4 per cell
 #   P2G momentum : pos 12 + vel 12 + C 36 + mass 4 + density 4 per particle;
 #                  mass read 4 + momentum write 12 per cell
@@ -50,7 +55,9 @@ ALG_BYTES = {
     "update": (0, 44),
     "g2p": (72, 12),
 }
-STAGES = ["p2g_mass", "p2g_mom", "update", "g2p", "diffuse", "sort", "step"]
+STAGES = ["p2g_mass", "p2g_mom", "update", "g2p", "diffuse", "sort", "step", "relayout"]
+KERNEL = {"g2p": "k_g2p_brick", "p2g_mom": "k_p2g_mom_brick", "update": "k_resolve_update",
+          "p2g_mass": "k_p2g_mass_brick + k_brick_list + k_resolve_mass"}
 
 
 def workload(name: str, seed: int):
@@ -106,7 +113,7 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--workload", default="dam-large", choices=["dam-large", "dam-small", "two-block"])
     ap.add_argument("--sort-period", type=int, default=1)
     ap.add_argument("--flags", type=int, default=0)
@@ -118,7 +125,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-events", action="store_true",
-                    help="time the region by wall clock only (no per-stage events)")
+                    help="skip the event-timed region (no stage times, no roofline)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 3 device-to-device copies of BYTES each (the plain copy that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -126,6 +136,18 @@ def main() -> int:
               "(multi-GPU sharding is a design note, not built). Run with --gpus 1.",
               file=sys.stderr)
         return 2
+
+    if args.copy_calibration:
+        import torch
+
+        nb = args.copy_calibration
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nb, "copies": 3}), flush=True)
+        return 0
 
     import mlsmpm
 
@@ -141,24 +163,31 @@ def main() -> int:
     s.sync()
     g0 = int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"]))
 
-    s.enable_timing(not args.no_events)
+    # Region 1: throughput, wall clock only.
+    s.enable_timing(False)
     s.sync()
     t0 = time.perf_counter()
     s.step(args.steps)
     s.sync()
     t1 = time.perf_counter()
-    stage = s.stage_times()
-    step_ms = s.step_times()
-    g1 = int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"]))
-    tstats = s.tile_stats()
-    s.enable_timing(False)
     elapsed = t1 - t0
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    # Region 2: the same number of steps with HIP events around every stage.
+    stage = np.zeros(len(STAGES), np.float32)
+    step_ms = np.zeros(0, np.float32)
+    if not args.no_events:
+        s.enable_timing(True)
+        s.step(args.steps)
+        stage = s.stage_times()
+        step_ms = s.step_times()
+        s.enable_timing(False)
+    g1 = int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"]))
+    tstats = s.tile_stats()
 
     g_act = 0.5 * (g0 + g1)
     ms = {k: float(v) for k, v in zip(STAGES, stage)}
     # dominant kernel and its roofline
-    dom = max(ALG_BYTES, key=lambda k: ms[k])
+    dom = max(("g2p", "p2g_mom", "update"), key=lambda k: ms[k])
     bp, bc = ALG_BYTES[dom]
     alg = bp * n + bc * g_act
     achieved = alg / (ms[dom] * 1e-3) if ms[dom] > 0 else 0.0
@@ -197,7 +226,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": KERNEL[dom],
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
@@ -209,6 +238,7 @@ def main() -> int:
         "step_roofline_frac": step_alg / (step_ms_mean * 1e-3) / HBM_PEAK,
         "stage_ms": ms,
         "tile_stats": tstats,
+        "events_ms_per_step": float(ms["step"]),
         "step_latency_ms": {
             "p50": float(np.percentile(step_ms, 50)) if len(step_ms) else None,
             "p99": float(np.percentile(step_ms, 99)) if len(step_ms) else None,

@@ -160,13 +160,16 @@ int mpm_set_walls(mpm_scene *s, const float *wallp84);
 int mpm_get_stream(mpm_scene *s, void **hip_stream);
 
 /* Stage timing with HIP events recorded on the scene's stream around every
- * stage of every step.  mpm_enable_timing(s, 1) clears the record and starts
- * it; mpm_stage_times gives the mean per step, in milliseconds, of
- * [p2g_mass, p2g_mom, update, g2p, diffuse, sort, whole step] over the steps
- * recorded since; mpm_step_times gives each recorded step's total (up to cap
- * values; *count = number recorded).  Both synchronise. */
+ * stage of every step (inside captured graphs as event-record nodes).
+ * mpm_enable_timing(s, 1) clears the record and starts it; mpm_stage_times
+ * gives the mean per step, in milliseconds, of
+ * [p2g_mass, p2g_mom, update, g2p, diffuse, sort, whole step, relayout] over
+ * the steps recorded since ("g2p" brackets the G2P kernel alone, "relayout"
+ * the offsets/work-item/permutation kernels of a re-sort); mpm_step_times
+ * gives each recorded step's total (up to cap values; *count = number
+ * recorded).  Both synchronise. */
 int mpm_enable_timing(mpm_scene *s, int enable);
-int mpm_stage_times(mpm_scene *s, float *ms7);
+int mpm_stage_times(mpm_scene *s, float *ms8);
 int mpm_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count);
 
 /* Diagnostics of the LDS-staged scatter: out[0] = work items of the current

@@ -224,9 +224,9 @@ int mpm_enable_timing(mpm_scene *s, int enable) {
   return scene_enable_timing(s, enable != 0);
 }
 
-int mpm_stage_times(mpm_scene *s, float *ms7) {
-  if (!s || !ms7) return fail(MPM_ERR_INVALID, "mpm_stage_times: NULL argument");
-  return scene_stage_times(s, ms7);
+int mpm_stage_times(mpm_scene *s, float *ms8) {
+  if (!s || !ms8) return fail(MPM_ERR_INVALID, "mpm_stage_times: NULL argument");
+  return scene_stage_times(s, ms8);
 }
 
 int mpm_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count) {

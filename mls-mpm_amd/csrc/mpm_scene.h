@@ -36,25 +36,33 @@ struct mpm_scene {
   // Stage timing: per step, events bracket sort, P2G mass, P2G momentum,
   // grid update, G2P and diffuse; folded into sums when the pool fills.
   bool timing = false;
-  static constexpr int kEvPerStep = 7;
+  static constexpr int kEvPerStep = 8;
   static constexpr int kEvSteps = 64;
   hipEvent_t ev[kEvPerStep * kEvSteps] = {};
   int ev_used = 0;               // steps recorded in the pool, not yet folded
-  double stage_sum[7] = {};      // ms: p2g_mass, p2g_mom, update, g2p, diffuse, sort, total
+  double stage_sum[8] = {};      // ms: p2g_mass, p2g_mom, update, g2p, diffuse, sort, total, relayout
   uint64_t timed_steps = 0;
   std::vector<float> step_ms;    // per-step totals since timing was enabled
-  // Captured multi-step graphs, valid when a step starts from (graph_cur,
-  // graph_since).  Untimed: graph[0] only.  Timed: graph[0] and graph[1]
-  // record their steps' events into the two halves of ev[] and are replayed
-  // alternately, so one replay's events are read while the other runs.
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
-  bool graph_timed = false;
-  int graph_next = 0;            // timed: which graph replays next
-  int graph_pending = -1;        // timed: replay whose events are not yet folded
-  int graph_cur = 0;
-  uint32_t graph_since = 0, graph_len = 0;
-  int post_cur = 0;
-  uint32_t post_since = 0;
+  // Captured multi-step graphs, one set per start state (buffer parity and
+  // steps since the last re-sort), least recently used replaced.  Untimed:
+  // exec[0] only.  Timed: exec[0] and exec[1] record their steps' events into
+  // the two halves of ev[] and are replayed alternately, so one replay's
+  // events are read while the other runs.
+  struct GraphSet {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool timed = false;
+    int cur = 0;
+    uint32_t since = 0;
+    int post_cur = 0;
+    uint32_t post_since = 0;
+    uint64_t used = 0;
+  };
+  static constexpr int kGraphSets = 4;
+  GraphSet gset[kGraphSets];
+  uint32_t graph_len = 0;        // steps per graph
+  uint64_t graph_clock = 0;
+  int graph_next = 0;            // timed: event half the next replay uses
+  int graph_pending = -1;        // timed: event half not yet folded
   size_t bytes = 0;         // device bytes held by the scene
   void *owned[64] = {};     // allocations to free
   int nowned = 0;
@@ -76,7 +84,7 @@ int scene_write_diffuse(mpm_scene *s, const void *rec48, const uint32_t *free_li
 int scene_set_seed(mpm_scene *s, uint32_t seed);
 int scene_set_walls(mpm_scene *s, const float *wallp);
 int scene_enable_timing(mpm_scene *s, bool on);
-int scene_stage_times(mpm_scene *s, float *ms7);
+int scene_stage_times(mpm_scene *s, float *ms8);
 int scene_tile_stats(mpm_scene *s, uint64_t out[4]);
 void scene_drop_graph(mpm_scene *s);
 int scene_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count);

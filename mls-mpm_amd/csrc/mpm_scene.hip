@@ -420,20 +420,21 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
   return MPM_OK;
 }
 
-// Add one step's 7 events (start, sort, P2G mass, P2G momentum, update, G2P,
-// diffuse) to the stage sums.
+// Add one step's 8 events (start, sort, P2G mass, P2G momentum, update,
+// relayout, G2P, diffuse) to the stage sums.
 static int fold_step_events(mpm_scene *s, hipEvent_t *e) {
   const int E = mpm_scene::kEvPerStep;
-  float ms[6];
-  for (int i = 0; i < 6; i++) HIPCHK(hipEventElapsedTime(&ms[i], e[i], e[i + 1]));
+  float ms[E - 1];
+  for (int i = 0; i < E - 1; i++) HIPCHK(hipEventElapsedTime(&ms[i], e[i], e[i + 1]));
   float tot;
   HIPCHK(hipEventElapsedTime(&tot, e[0], e[E - 1]));
   s->stage_sum[5] += ms[0];
   s->stage_sum[0] += ms[1];
   s->stage_sum[1] += ms[2];
   s->stage_sum[2] += ms[3];
-  s->stage_sum[3] += ms[4];
-  s->stage_sum[4] += ms[5];
+  s->stage_sum[7] += ms[4];
+  s->stage_sum[3] += ms[5];
+  s->stage_sum[4] += ms[6];
   s->stage_sum[6] += tot;
   s->step_ms.push_back(tot);
   s->timed_steps++;
@@ -462,8 +463,8 @@ static int fold_timing(mpm_scene *s) {
   return fold_graph_events(s);
 }
 
-// Enqueue one step.  e (optional) receives 7 events: start, after the sort,
-// P2G mass, P2G momentum, grid update, G2P, diffuse.
+// Enqueue one step.  e (optional) receives 8 events: start, after the sort,
+// P2G mass, P2G momentum, grid update, relayout preparation, G2P, diffuse.
 // When captured (graph_events), the events become event-record nodes.
 static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) {
   const bool keep = (s->cfg.flags & MPM_FLAG_KEEP_GRID) != 0;
@@ -514,8 +515,9 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   if (tiled) launch_resolve_update(P, s->g, s->tc, s->bw, s->walls, keep, st);
   else launch_update_dense(P, s->g, s->walls, !keep, st);
   if ((rc = mark(4))) return rc;
+  if (relayout) launch_relayout_prep(P, s->sort, s->bw, s->tc, st);
+  if ((rc = mark(5))) return rc;
   if (relayout) {
-    launch_relayout_prep(P, s->sort, s->bw, s->tc, st);
     PState dst = pstate(s, s->cur ^ 1);
     launch_g2p_tiled(P, S, dst, s->g, s->walls, s->tc, s->bw, s->sort.src_of, st);
     s->cur ^= 1;
@@ -526,20 +528,33 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   } else {
     launch_g2p_global(P, S, s->g, s->walls, st);
   }
-  if ((rc = mark(5))) return rc;
-  launch_diffuse(P, S, s->g, s->walls, s->d, st);
   if ((rc = mark(6))) return rc;
+  launch_diffuse(P, S, s->g, s->walls, s->d, st);
+  if ((rc = mark(7))) return rc;
   HIPCHK(hipGetLastError());
   s->steps++;
   s->since_sort++;
   return MPM_OK;
 }
 
-void scene_drop_graph(mpm_scene *s) {
-  for (hipGraphExec_t &g : s->graph) {
+static void drop_set(mpm_scene::GraphSet &gs) {
+  for (hipGraphExec_t &g : gs.exec) {
     if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
   }
+}
+
+// Releases every captured graph after the stream has drained (a replay may
+// still run) and the pending timed replay's events are folded.
+void scene_drop_graph(mpm_scene *s) {
+  bool any = false;
+  for (const mpm_scene::GraphSet &gs : s->gset) any = any || gs.exec[0];
+  if (any && s->stream) {
+    (void)hipSetDevice(s->cfg.device);
+    (void)hipStreamSynchronize(s->stream);
+    (void)fold_graph_events(s);
+  }
+  for (mpm_scene::GraphSet &gs : s->gset) drop_set(gs);
   s->graph_len = 0;
   s->graph_next = 0;
   s->graph_pending = -1;
@@ -554,9 +569,9 @@ static uint32_t graph_steps(uint32_t period) {
   return cyc * k;
 }
 
-// Capture graph_steps() steps from the current layout state into graph[slot];
-// timed graphs record their events into ev[slot * L * kEvPerStep ...].
-static int capture_one(mpm_scene *s, int slot, bool timed) {
+// Capture graph_steps() steps from the current layout state into gs.exec[half];
+// timed graphs record their events into ev[half * L * kEvPerStep ...].
+static int capture_one(mpm_scene *s, mpm_scene::GraphSet &gs, int half, bool timed) {
   const int cur0 = s->cur;
   const uint32_t since0 = s->since_sort;
   const uint64_t steps0 = s->steps;
@@ -564,13 +579,13 @@ static int capture_one(mpm_scene *s, int slot, bool timed) {
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = MPM_OK;
   for (uint32_t i = 0; i < L && !rc; i++) {
-    hipEvent_t *e = timed ? &s->ev[((size_t)slot * L + i) * mpm_scene::kEvPerStep] : nullptr;
+    hipEvent_t *e = timed ? &s->ev[((size_t)half * L + i) * mpm_scene::kEvPerStep] : nullptr;
     rc = enqueue_step(s, e, true);
   }
   hipGraph_t g = nullptr;
   hipError_t err = hipStreamEndCapture(s->stream, &g);
-  s->post_cur = s->cur;
-  s->post_since = s->since_sort;
+  gs.post_cur = s->cur;
+  gs.post_since = s->since_sort;
   // capture only recorded the work: put the host state back
   s->cur = cur0;
   s->since_sort = since0;
@@ -580,28 +595,44 @@ static int capture_one(mpm_scene *s, int slot, bool timed) {
     return rc;
   }
   if (err != hipSuccess) return hip_fail(err, "hipStreamEndCapture");
-  err = hipGraphInstantiate(&s->graph[slot], g, nullptr, nullptr, 0);
+  err = hipGraphInstantiate(&gs.exec[half], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (err != hipSuccess) {
-    s->graph[slot] = nullptr;
+    gs.exec[half] = nullptr;
     return hip_fail(err, "hipGraphInstantiate");
   }
   return MPM_OK;
 }
 
-static int capture_graphs(mpm_scene *s, bool timed) {
-  scene_drop_graph(s);
-  for (int slot = 0; slot < (timed ? 2 : 1); slot++) {
-    int rc = capture_one(s, slot, timed);
+// The graph set for the current start state, captured if absent.
+static int find_graphs(mpm_scene *s, mpm_scene::GraphSet **out) {
+  mpm_scene::GraphSet *lru = &s->gset[0];
+  for (mpm_scene::GraphSet &gs : s->gset) {
+    if (gs.exec[0] && gs.timed == s->timing && gs.cur == s->cur && gs.since == s->since_sort) {
+      gs.used = ++s->graph_clock;
+      *out = &gs;
+      return MPM_OK;
+    }
+    if (!gs.exec[0] || gs.used < lru->used) lru = &gs;
+  }
+  if (lru->exec[0]) {
+    // the replaced graphs may still run
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drop_set(*lru);
+  }
+  for (int half = 0; half < (s->timing ? 2 : 1); half++) {
+    int rc = capture_one(s, *lru, half, s->timing);
     if (rc) {
-      scene_drop_graph(s);
+      drop_set(*lru);
       return rc;
     }
   }
-  s->graph_timed = timed;
-  s->graph_cur = s->cur;
-  s->graph_since = s->since_sort;
+  lru->timed = s->timing;
+  lru->cur = s->cur;
+  lru->since = s->since_sort;
+  lru->used = ++s->graph_clock;
   s->graph_len = graph_steps(s->cfg.sort_period);
+  *out = lru;
   return MPM_OK;
 }
 
@@ -629,28 +660,23 @@ int scene_step(mpm_scene *s, uint32_t nsteps) {
     const bool graphable = L > 0 && !(s->cfg.flags & MPM_FLAG_NO_GRAPH) && tiled && s->sorted_once &&
                            s->n > 0 && (!s->timing || 2 * L <= (uint32_t)mpm_scene::kEvSteps);
     if (graphable && nsteps >= L) {
-      if (!s->graph[0] || s->graph_timed != s->timing || s->graph_cur != s->cur ||
-          s->graph_since != s->since_sort) {
-        int rc = fold_timing(s);
-        if (rc) return rc;
-        if ((rc = capture_graphs(s, s->timing))) return rc;
-      }
-      int slot = 0;
+      // eager-step events share the pool with the graphs' events
+      int rc = fold_eager(s);
+      if (rc) return rc;
+      mpm_scene::GraphSet *gs = nullptr;
+      if ((rc = find_graphs(s, &gs))) return rc;
       if (s->timing) {
-        // eager-step events share the pool with the graphs' events
-        int rc = fold_eager(s);
-        if (rc) return rc;
-        slot = s->graph_next;
-        HIPCHK(hipGraphLaunch(s->graph[slot], s->stream));
-        // the other graph's replay (if any) was enqueued before this one
+        const int half = s->graph_next;
+        HIPCHK(hipGraphLaunch(gs->exec[half], s->stream));
+        // the replay that used the other half was enqueued before this one
         if ((rc = fold_graph_events(s))) return rc;
-        s->graph_pending = slot;
-        s->graph_next = slot ^ 1;
+        s->graph_pending = half;
+        s->graph_next = half ^ 1;
       } else {
-        HIPCHK(hipGraphLaunch(s->graph[0], s->stream));
+        HIPCHK(hipGraphLaunch(gs->exec[0], s->stream));
       }
-      s->cur = s->post_cur;
-      s->since_sort = s->post_since;
+      s->cur = gs->post_cur;
+      s->since_sort = gs->post_since;
       s->steps += s->graph_len;
       nsteps -= s->graph_len;
       continue;
@@ -792,12 +818,12 @@ int scene_enable_timing(mpm_scene *s, bool on) {
   return MPM_OK;
 }
 
-int scene_stage_times(mpm_scene *s, float *ms7) {
+int scene_stage_times(mpm_scene *s, float *ms8) {
   HIPCHK(hipSetDevice(s->cfg.device));
   int rc = fold_timing(s);
   if (rc) return rc;
-  for (int i = 0; i < 7; i++)
-    ms7[i] = s->timed_steps ? (float)(s->stage_sum[i] / (double)s->timed_steps) : 0.0f;
+  for (int i = 0; i < 8; i++)
+    ms8[i] = s->timed_steps ? (float)(s->stage_sum[i] / (double)s->timed_steps) : 0.0f;
   return MPM_OK;
 }
 

@@ -37,6 +37,7 @@
 namespace mpm {
 
 constexpr int kTB = 256;    // row width of an item (slots per row)
+constexpr int kMaxPpt = 16; // largest particles-per-thread of a work item
 constexpr int kCore = 10;   // brick + 1-cell halo per axis
 constexpr int kCoreV = kCore * kCore * kCore;
 constexpr uint32_t kFar = 31u;  // rank tag: particle left the 27-brick neighbourhood
@@ -233,13 +234,15 @@ __global__ __launch_bounds__(256) void k_p2g_mom_brick(Params P, PState S, GStat
   __syncthreads();
   const uint32_t gx = P.gx, slice = P.gx * P.gy;
   // Software pipeline: the next particle's inputs load while this one computes.
+  // The prefetch index past the item's end is clamped to its last particle
+  // (in bounds, unconditional).
   uint32_t idx = t;
   MomIn cur;
   if (idx < it.z) load_mom_in<UNIT>(S, it.y + idx, cur);
   while (idx < it.z) {
     const uint32_t s = it.y + idx, nidx = idx + nt;
     MomIn nxt;
-    if (nidx < it.z) load_mom_in<UNIT>(S, it.y + nidx, nxt);
+    load_mom_in<UNIT>(S, it.y + (nidx < it.z ? nidx : it.z - 1), nxt);
     PW pw;
     particle_weights(cur.px, cur.py, cur.pz, pw);
     WinPos wp = win_pos<M>(pw, ox, oy, oz);
@@ -453,6 +456,8 @@ __global__ __launch_bounds__(256) void k_g2p_brick(Params P, PState S, PState D,
                                                    const uint32_t *src_of) {
   constexpr int E = 10 + 2 * M, V = E * E * E;
   __shared__ float4 lv[V];
+  // Re-sort: the item's source slots, staged with the window.
+  __shared__ uint32_t lsrc[EXACT ? kTB * kMaxPpt : 1];
   if (blockIdx.x >= *bw.nitems) return;
   const uint4 it = bw.items[blockIdx.x];
   int ibx, iby, ibz;
@@ -463,28 +468,37 @@ __global__ __launch_bounds__(256) void k_g2p_brick(Params P, PState S, PState D,
     int gi = window_cell<E>(P, ox, oy, oz, c);
     lv[c] = gi >= 0 ? g.v[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
+  if (EXACT)
+    for (uint32_t i = t; i < it.z; i += nt) lsrc[i] = src_of[it.y + i];
   __syncthreads();
+  uint32_t idx = t;
+  if (idx >= it.z) return;
   const uint32_t gx = P.gx, slice = P.gx * P.gy;
-  // Software pipeline: the next particle's source slot and inputs load while
-  // this one computes.
+  // Software pipeline: the next particle's inputs (for a re-sort also the
+  // mass, id and density it moves) load while this one computes.  Prefetch
+  // indexes past the item's end are clamped to its last particle, so every
+  // load is in bounds and unconditional.  The first particle is peeled off
+  // the loop: vmcnt counts loads and stores in issue order, and with both
+  // paths into the loop alike (loads, then the previous particle's stores)
+  // the wait for a particle's inputs does not cover the stores before it.
   struct In {
-    uint32_t s;
-    float px, py, pz;
+    uint32_t s, id;
+    float px, py, pz, m, dens;
   };
-  auto load_in = [&](uint32_t d, In &in) {
-    in.s = EXACT ? src_of[d] : d;
+  auto load_in = [&](uint32_t i, In &in) {
+    const uint32_t j = i < it.z ? i : it.z - 1;
+    in.s = EXACT ? lsrc[j] : it.y + j;
     in.px = S.at(PX, in.s);
     in.py = S.at(PY, in.s);
     in.pz = S.at(PZ, in.s);
+    if (EXACT) {
+      in.m = S.at(PM, in.s);
+      in.id = S.id[in.s];
+      in.dens = S.density[in.s];
+    }
   };
-  uint32_t idx = t;
-  In cur;
-  if (idx < it.z) load_in(it.y + idx, cur);
-  while (idx < it.z) {
-    const uint32_t d = it.y + idx, nidx = idx + nt;
-    In nxt;
-    if (nidx < it.z) load_in(it.y + nidx, nxt);
-    const uint32_t s = cur.s;
+  auto body = [&](uint32_t i, const In &cur) {
+    const uint32_t d = it.y + i;
     const float px = cur.px, py = cur.py, pz = cur.pz;
     PW pw;
     particle_weights(px, py, pz, pw);
@@ -492,25 +506,32 @@ __global__ __launch_bounds__(256) void k_g2p_brick(Params P, PState S, PState D,
     G2POut o;
     if (EXACT || wp.inside) {
       const int lb = wp.lb;
-      g2p_particle(P, W, px, py, pz, pw, [&](int kk, int j, int i) {
-        float4 v = lv[lb + (kk * E + j) * E + i];
+      g2p_particle(P, W, px, py, pz, pw, [&](int kk, int j, int ii) {
+        float4 v = lv[lb + (kk * E + j) * E + ii];
         return make_float3(v.x, v.y, v.z);
       }, o);
     } else {
       uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
-      g2p_particle(P, W, px, py, pz, pw, [&](int kk, int j, int i) {
-        float4 v = g.v[base + kk * slice + j * gx + i];
+      g2p_particle(P, W, px, py, pz, pw, [&](int kk, int j, int ii) {
+        float4 v = g.v[base + kk * slice + j * gx + ii];
         return make_float3(v.x, v.y, v.z);
       }, o);
     }
     store_g2p(D, d, o);
     if (EXACT) {
-      D.at(PM, d) = S.at(PM, s);
-      D.id[d] = S.id[s];
-      D.density[d] = S.density[s];
+      D.at(PM, d) = cur.m;
+      D.id[d] = cur.id;
+      D.density[d] = cur.dens;
     }
+  };
+  In cur, nxt;
+  load_in(idx, cur);
+  load_in(idx + nt, nxt);
+  body(idx, cur);
+  for (idx += nt; idx < it.z; idx += nt) {
     cur = nxt;
-    idx = nidx;
+    load_in(idx + nt, nxt);
+    body(idx, cur);
   }
 }
 

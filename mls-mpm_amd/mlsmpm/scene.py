@@ -153,7 +153,7 @@ class Scene:
         check(load().mpm_enable_timing(self._h, 1 if on else 0))
 
     def stage_times(self) -> np.ndarray:
-        ms = np.zeros(7, np.float32)
+        ms = np.zeros(8, np.float32)
         check(load().mpm_stage_times(self._h, _p(ms)))
         return ms
 
