@@ -177,6 +177,55 @@ int mpm_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count);
  * out[3] = largest item in particles.  Synchronises. */
 int mpm_tile_stats(mpm_scene *s, uint64_t out[4]);
 
+/* ------------------------------------------------------------------------
+ * Batched form.  No counterpart in the reference, which steps one FluidGPU
+ * per WebGPU device: a batch steps many scenes of uneven particle counts and
+ * grid sizes with one persistent launch per stage, captured in a HIP graph.
+ * Each member's result is bit-identical to stepping it alone.
+ *   - Members re-sort every step (sort_period 1) on the LDS-staged path,
+ *     carry no diffuse pool, share the tile margin and live on the batch's
+ *     device; their work runs on the batch's stream while they are members
+ *     (every single-scene call on a member stays valid and is ordered with
+ *     the batch).
+ *   - A member not yet stepped since its reset takes that first step (the
+ *     sort of the caller's order) through the single-scene path, together
+ *     with the other members.
+ *   - mpm_destroy on a member removes it from its batch first.
+ */
+typedef struct mpm_batch mpm_batch;
+
+#define MPM_BATCH_NO_GRAPH 1u  /* mpm_batch_tuning.flags: enqueue every step eagerly */
+
+/* Launch shapes of the batched kernels, as chosen by the autotuner
+ * (mlsmpm.autotune): persistent blocks of the work-item kernels (P2G mass,
+ * P2G momentum, G2P), of the resolve kernels and of the per-brick /
+ * per-particle kernels (0 = one block per unit), and steps per captured
+ * graph (rounded up to even; 0 = 16). */
+typedef struct mpm_batch_tuning {
+    uint32_t item_blocks;
+    uint32_t resolve_blocks;
+    uint32_t aux_blocks;
+    uint32_t graph_steps;
+    uint32_t flags;
+    uint32_t reserved[3];
+} mpm_batch_tuning;
+
+int mpm_batch_create(int device, mpm_batch **out);
+/* Removes every member (they return to their own streams), then frees. */
+int mpm_batch_destroy(mpm_batch *b);
+int mpm_batch_add(mpm_batch *b, mpm_scene *s);
+int mpm_batch_remove(mpm_batch *b, mpm_scene *s);
+int mpm_batch_count(const mpm_batch *b, uint32_t *count);
+int mpm_batch_set_tuning(mpm_batch *b, const mpm_batch_tuning *t);
+int mpm_batch_get_tuning(const mpm_batch *b, mpm_batch_tuning *t);
+/* Enqueue nsteps steps of every member (each member advances nsteps). */
+int mpm_batch_step(mpm_batch *b, uint32_t nsteps);
+int mpm_batch_sync(mpm_batch *b);
+/* Per batched step GPU time from HIP events (timing disables graph capture);
+ * mpm_batch_step_times synchronises and returns up to cap values. */
+int mpm_batch_enable_timing(mpm_batch *b, int enable);
+int mpm_batch_step_times(mpm_batch *b, float *ms, uint32_t cap, uint32_t *count);
+
 #ifdef __cplusplus
 }
 #endif

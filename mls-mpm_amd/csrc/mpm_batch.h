@@ -1,0 +1,60 @@
+// mpm_batch.h -- many scenes stepped together: per-scene device descriptors
+// and the batched stage launchers (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/mlsmpm.h"
+#include "mpm_kernels.h"
+#include "mpm_tiled.h"
+
+namespace mpm {
+
+// Everything one scene's stage bodies need, for one buffer parity: the
+// batched kernels read it from a device table indexed by scene.
+struct SceneDev {
+  Params P;
+  PState S;            // current particle buffer
+  PState D;            // re-sort destination buffer
+  GState g;
+  BrickWork bw;
+  KeyOut ko;           // brick counts, keys and ranks recorded by P2G mass
+  uint32_t *offsets;   // per brick rank: first particle of the new order
+  uint32_t *src_of;    // per new slot: old slot
+  uint32_t ch;         // work-item size in particles (256 * ppt)
+  int32_t keep;        // keep the i32 momentum grids readable
+  Walls W;
+};
+
+// The work units of one stage over all scenes: unit u belongs to the scene
+// si with pre[si] <= u < pre[si + 1].
+struct BatchView {
+  const SceneDev *sd;
+  const uint32_t *pre;
+  uint32_t ns, total;
+};
+
+// Persistent blocks of each batched kernel class (0 = one block per unit).
+struct BatchGrid {
+  uint32_t items, resolve, aux;
+};
+
+// Per-stage unit prefixes (device, ns + 1 words each) of a batch table.
+struct BatchTables {
+  const SceneDev *sd;
+  const uint32_t *pre_items;    // work items (upper bound per scene)
+  const uint32_t *pre_bricks;   // bricks (upper bound of the resolve list)
+  const uint32_t *pre_bblocks;  // 256-thread blocks over bricks
+  const uint32_t *pre_pblocks;  // 256-thread blocks over particles
+  uint32_t ns;
+  uint32_t tot_items, tot_bricks, tot_bblocks, tot_pblocks;
+  uint32_t all_unit;            // every scene has unit particle mass
+};
+
+// One batched step (sort period 1: every step re-sorts inside G2P) of every
+// scene of the table, all with drift margin `margin`.
+void launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &grid,
+                       hipStream_t st);
+
+}  // namespace mpm

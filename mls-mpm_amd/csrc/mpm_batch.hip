@@ -1,0 +1,587 @@
+// mpm_batch.hip -- many independent scenes stepped by one launch per stage.
+//
+// The reference steps one fluid per WebGPU device (FluidGPU.simFrame,
+// fluid-gpu.js:648-684).  Here a batch holds any number of scenes of uneven
+// particle counts and grid sizes and runs each stage of the step as one
+// persistent kernel over all of them: a per-scene descriptor table (one per
+// buffer parity) and, per stage, a prefix over the scenes' work units
+// (work items, bricks, particle blocks) map every unit to its scene, and the
+// unit runs the same device body as the single-scene kernel
+// (mpm_tiled_dev.h), so every scene's result is bit-identical to stepping it
+// alone.  Steps are captured in a HIP graph of an even number of steps (the
+// buffer parity returns), replayed while membership and scene states are
+// unchanged.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "mpm_batch.h"
+#include "mpm_scene.h"
+#include "mpm_tiled_dev.h"
+
+namespace mpm {
+
+// Scene of unit u: the largest si with pre[si] <= u (pre[0] = 0 <= u < pre[ns]).
+__device__ __forceinline__ uint32_t find_scene(const uint32_t *__restrict__ pre, uint32_t ns,
+                                               uint32_t u) {
+  uint32_t lo = 0, hi = ns;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_batch_zero(const SceneDev *__restrict__ sd, uint32_t ns) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ns) *sd[i].bw.nlist = 0u;
+}
+
+// UNIT: every scene of the table has unit particle mass (the general body
+// gives the same bits for unit mass, multiplying by 1.0f exactly).
+template <int M, bool UNIT>
+__global__ __launch_bounds__(256) void k_batch_mass(const SceneDev *__restrict__ sd,
+                                                    const uint32_t *__restrict__ pre, uint32_t ns,
+                                                    uint32_t total) {
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t si = find_scene(pre, ns, u);
+    const SceneDev &d = sd[si];
+    const uint32_t blk = u - pre[si];
+    p2g_mass_item<M, UNIT>(d.P, d.S, d.bw, d.ko, blk);
+    __syncthreads();
+  }
+}
+
+// UNIT: every scene of the table has unit particle mass (the general body
+// gives the same bits for unit mass, multiplying by 1.0f exactly).
+template <int M, bool UNIT>
+__global__ __launch_bounds__(256) void k_batch_mom(const SceneDev *__restrict__ sd,
+                                                   const uint32_t *__restrict__ pre, uint32_t ns,
+                                                   uint32_t total) {
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t si = find_scene(pre, ns, u);
+    const SceneDev &d = sd[si];
+    const uint32_t blk = u - pre[si];
+    p2g_mom_item<M, UNIT>(d.P, d.S, d.g, d.bw, blk);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_g2p(const SceneDev *__restrict__ sd,
+                                                   const uint32_t *__restrict__ pre, uint32_t ns,
+                                                   uint32_t total) {
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t si = find_scene(pre, ns, u);
+    const SceneDev &d = sd[si];
+    g2p_item<0, true>(d.P, d.S, d.D, d.g, d.W, d.bw, d.src_of, u - pre[si]);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_brick_list(const SceneDev *__restrict__ sd,
+                                                          const uint32_t *__restrict__ pre,
+                                                          uint32_t ns, uint32_t total) {
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t si = find_scene(pre, ns, u);
+    brick_list_one(sd[si].bw, (u - pre[si]) * 256u + threadIdx.x);
+  }
+}
+
+__global__ __launch_bounds__(512) void k_batch_resolve_mass(const SceneDev *__restrict__ sd,
+                                                            const uint32_t *__restrict__ pre,
+                                                            uint32_t ns, uint32_t total) {
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t si = find_scene(pre, ns, u);
+    const SceneDev &d = sd[si];
+    const uint32_t i = u - pre[si];
+    if (i < *d.bw.nlist) resolve_mass_entry(d.P, d.g, d.bw, i);
+  }
+}
+
+__global__ __launch_bounds__(512) void k_batch_resolve_update(const SceneDev *__restrict__ sd,
+                                                              const uint32_t *__restrict__ pre,
+                                                              uint32_t ns, uint32_t total) {
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t si = find_scene(pre, ns, u);
+    const SceneDev &d = sd[si];
+    const uint32_t i = u - pre[si];
+    if (i < *d.bw.nlist) resolve_update_entry(d.P, d.g, d.bw, d.W, d.keep, i);
+  }
+}
+
+// Per scene (one block each, persistent): offsets = exclusive scan of the
+// brick counts, counts cleared for the next step, work-item count reset.
+__global__ __launch_bounds__(256) void k_batch_scan(const SceneDev *__restrict__ sd, uint32_t ns) {
+  __shared__ uint32_t wsum[4];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  for (uint32_t si = blockIdx.x; si < ns; si += gridDim.x) {
+    const SceneDev &d = sd[si];
+    const uint32_t n = d.bw.nbricks;
+    uint32_t *counts = d.ko.counts, *offsets = d.offsets;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += 1024u) {
+      const uint32_t i0 = base + t * 4u;
+      uint32_t a[4], sum = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        a[k] = (i0 + k < n) ? counts[i0 + k] : 0u;
+        sum += a[k];
+      }
+      uint32_t incl = sum;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += v;
+      }
+      if (lane == 63u) wsum[wave] = incl;
+      __syncthreads();
+      uint32_t woff = 0, tot = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < 4; w++) {
+        if (w < wave) woff += wsum[w];
+        tot += wsum[w];
+      }
+      uint32_t e = carry + woff + incl - sum;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (i0 + k < n) {
+          offsets[i0 + k] = e;
+          counts[i0 + k] = 0u;
+        }
+        e += a[k];
+      }
+      carry += tot;
+      __syncthreads();
+    }
+    if (t == 0) *d.bw.nitems = 0u;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_make_items(const SceneDev *__restrict__ sd,
+                                                          const uint32_t *__restrict__ pre,
+                                                          uint32_t ns, uint32_t total) {
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t si = find_scene(pre, ns, u);
+    const SceneDev &d = sd[si];
+    make_items_one(d.P.n, d.bw, d.ch, d.offsets, (u - pre[si]) * 256u + threadIdx.x);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_build_src(const SceneDev *__restrict__ sd,
+                                                         const uint32_t *__restrict__ pre,
+                                                         uint32_t ns, uint32_t total) {
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t si = find_scene(pre, ns, u);
+    const SceneDev &d = sd[si];
+    build_src_one(d.P.n, d.bw.nbricks, d.ch, d.ko.keys, d.ko.rank, d.offsets, d.src_of,
+                  (u - pre[si]) * 256u + threadIdx.x);
+  }
+}
+
+static inline uint32_t grid_of(uint32_t total, uint32_t cap) {
+  return (cap && cap < total) ? cap : total;
+}
+
+void launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &grid,
+                       hipStream_t st) {
+  const uint32_t ns = t.ns;
+  if (!ns) return;
+  k_batch_zero<<<(ns + 255) / 256, 256, 0, st>>>(t.sd, ns);
+  const uint32_t gi = grid_of(t.tot_items, grid.items);
+  const uint32_t gr = grid_of(t.tot_bricks, grid.resolve);
+  const uint32_t gb = grid_of(t.tot_bblocks, grid.aux);
+  const uint32_t gp = grid_of(t.tot_pblocks, grid.aux);
+#define MPM_BATCH_ITEMS(K)                                                         \
+  do {                                                                             \
+    if (margin == 1 && t.all_unit) K<1, true><<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items); \
+    else if (margin == 1) K<1, false><<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);         \
+    else if (t.all_unit) K<2, true><<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);           \
+    else K<2, false><<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);                         \
+  } while (0)
+  MPM_BATCH_ITEMS(k_batch_mass);
+  k_batch_brick_list<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks);
+  k_batch_resolve_mass<<<gr, 512, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
+  MPM_BATCH_ITEMS(k_batch_mom);
+#undef MPM_BATCH_ITEMS
+  k_batch_resolve_update<<<gr, 512, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
+  k_batch_scan<<<std::min<uint32_t>(ns, 2048), 256, 0, st>>>(t.sd, ns);
+  k_batch_make_items<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks);
+  k_batch_build_src<<<gp, 256, 0, st>>>(t.sd, t.pre_pblocks, ns, t.tot_pblocks);
+  k_batch_g2p<<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);
+}
+
+}  // namespace mpm
+
+using namespace mpm;
+
+// ---------------------------------------------------------------------------
+// The batch object.
+struct mpm_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mpm_batch_tuning tune{};
+  std::vector<mpm_scene *> mem;     // members
+  std::vector<uint64_t> seen;       // member versions the tables were built from
+  uint32_t margin = 0;              // drift margin shared by the members
+  bool dirty = true;
+  int phase = 0;                    // parity of batched steps since the tables were built
+  std::vector<mpm_scene *> active;  // members with particles, in table order
+  void *dev = nullptr;              // descriptor tables and prefixes
+  size_t dev_bytes = 0;
+  BatchTables tab[2]{};
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // by start phase
+  uint32_t graph_len = 0;
+  bool timing = false;
+  static constexpr uint32_t kEv = 128;  // batched steps per event pool
+  hipEvent_t ev[2 * kEv] = {};
+  uint32_t ev_used = 0;
+  std::vector<float> step_ms;
+};
+
+namespace {
+
+int bfail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int bfail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  return set_error(code, buf);
+}
+
+#define BHIP(call)                                    \
+  do {                                                \
+    hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+uint32_t cdiv32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+void drop_graphs(mpm_batch *b) {
+  for (hipGraphExec_t &g : b->graph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
+int fold_events(mpm_batch *b) {
+  for (uint32_t k = 0; k < b->ev_used; k++) {
+    BHIP(hipEventSynchronize(b->ev[2 * k + 1]));
+    float ms = 0.0f;
+    BHIP(hipEventElapsedTime(&ms, b->ev[2 * k], b->ev[2 * k + 1]));
+    b->step_ms.push_back(ms);
+  }
+  b->ev_used = 0;
+  return MPM_OK;
+}
+
+// Descriptor tables and unit prefixes of the current members.
+int build_tables(mpm_batch *b) {
+  BHIP(hipStreamSynchronize(b->stream));
+  drop_graphs(b);
+  b->active.clear();
+  for (mpm_scene *s : b->mem)
+    if (s->n) b->active.push_back(s);
+  const uint32_t ns = (uint32_t)b->active.size();
+  std::vector<SceneDev> h0(ns), h1(ns);
+  std::vector<uint32_t> pre(4 * (size_t)(ns + 1), 0u);
+  uint32_t all_unit = 1;
+  uint32_t *pi = pre.data(), *pb = pi + (ns + 1), *pbb = pb + (ns + 1), *ppb = pbb + (ns + 1);
+  for (uint32_t i = 0; i < ns; i++) {
+    mpm_scene *s = b->active[i];
+    scene_dev(s, 0, &h0[i]);
+    scene_dev(s, 1, &h1[i]);
+    all_unit &= h0[i].P.unit_mass ? 1u : 0u;
+    const uint32_t nb = s->bw.nbricks;
+    pi[i + 1] = pi[i] + max_items(s->n, nb, s->tc.ppt);
+    pb[i + 1] = pb[i] + nb;
+    pbb[i + 1] = pbb[i] + cdiv32(nb, 256);
+    ppb[i + 1] = ppb[i] + cdiv32(s->n, 256);
+  }
+  const size_t tbytes = (size_t)ns * sizeof(SceneDev);
+  const size_t need = 2 * tbytes + pre.size() * 4 + 256;
+  if (need > b->dev_bytes) {
+    if (b->dev) BHIP(hipFree(b->dev));
+    b->dev = nullptr;
+    b->dev_bytes = 0;
+    BHIP(hipMalloc(&b->dev, need));
+    b->dev_bytes = need;
+  }
+  char *base = (char *)b->dev;
+  SceneDev *d0 = (SceneDev *)base, *d1 = (SceneDev *)(base + tbytes);
+  uint32_t *dpre = (uint32_t *)(base + 2 * tbytes);
+  if (ns) {
+    BHIP(hipMemcpyAsync(d0, h0.data(), tbytes, hipMemcpyHostToDevice, b->stream));
+    BHIP(hipMemcpyAsync(d1, h1.data(), tbytes, hipMemcpyHostToDevice, b->stream));
+    BHIP(hipMemcpyAsync(dpre, pre.data(), pre.size() * 4, hipMemcpyHostToDevice, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+  }
+  for (int ph = 0; ph < 2; ph++) {
+    BatchTables &t = b->tab[ph];
+    t.sd = ph ? d1 : d0;
+    t.pre_items = dpre;
+    t.pre_bricks = dpre + (ns + 1);
+    t.pre_bblocks = dpre + 2 * (ns + 1);
+    t.pre_pblocks = dpre + 3 * (ns + 1);
+    t.ns = ns;
+    t.tot_items = pi[ns];
+    t.tot_bricks = pb[ns];
+    t.tot_bblocks = pbb[ns];
+    t.tot_pblocks = ppb[ns];
+    t.all_unit = all_unit;
+  }
+  b->seen.resize(b->mem.size());
+  for (size_t i = 0; i < b->mem.size(); i++) b->seen[i] = b->mem[i]->version;
+  b->phase = 0;
+  b->dirty = false;
+  return MPM_OK;
+}
+
+BatchGrid grid_of(const mpm_batch *b) {
+  return BatchGrid{b->tune.item_blocks, b->tune.resolve_blocks, b->tune.aux_blocks};
+}
+
+// Host side of k batched steps: every member advanced k steps.
+void advance(mpm_batch *b, uint32_t k) {
+  b->phase ^= (int)(k & 1u);
+  for (mpm_scene *s : b->mem) {
+    if (s->n) {
+      s->cur ^= (int)(k & 1u);
+      s->since_sort = 1;
+    }
+    s->steps += k;
+  }
+}
+
+uint32_t graph_len_of(const mpm_batch *b) {
+  uint32_t L = b->tune.graph_steps ? b->tune.graph_steps : 16u;
+  return L + (L & 1u);  // even: the buffer parity returns
+}
+
+int capture(mpm_batch *b, int ph) {
+  const uint32_t L = graph_len_of(b);
+  BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (uint32_t i = 0; i < L; i++)
+    launch_batch_step(b->tab[(ph + (int)i) & 1], b->margin, grid_of(b), b->stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&b->graph[ph], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->graph[ph] = nullptr;
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  b->graph_len = L;
+  return MPM_OK;
+}
+
+int remove_member(mpm_batch *b, mpm_scene *s) {
+  auto it = std::find(b->mem.begin(), b->mem.end(), s);
+  if (it == b->mem.end()) return bfail(MPM_ERR_NOT_FOUND, "scene is not a member of this batch");
+  BHIP(hipSetDevice(b->device));
+  BHIP(hipStreamSynchronize(b->stream));
+  s->stream = s->own_stream;
+  s->own_stream = nullptr;
+  s->batch = nullptr;
+  b->mem.erase(it);
+  b->dirty = true;
+  return MPM_OK;
+}
+
+}  // namespace
+
+namespace mpm {
+int batch_detach(mpm_scene *s) { return s->batch ? remove_member(s->batch, s) : MPM_OK; }
+}  // namespace mpm
+
+extern "C" {
+
+int mpm_batch_create(int device, mpm_batch **out) {
+  if (!out) return bfail(MPM_ERR_INVALID, "mpm_batch_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return bfail(MPM_ERR_HIP, "no HIP device available (%s)",
+                 e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+  if (device < 0 || device >= ndev)
+    return bfail(MPM_ERR_INVALID, "device %d out of range (%d devices)", device, ndev);
+  BHIP(hipSetDevice(device));
+  mpm_batch *b = new mpm_batch();
+  b->device = device;
+  b->tune.item_blocks = 0;
+  b->tune.resolve_blocks = 2048;
+  b->tune.aux_blocks = 2048;
+  b->tune.graph_steps = 16;
+  hipError_t ce = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  for (uint32_t i = 0; ce == hipSuccess && i < 2 * mpm_batch::kEv; i++)
+    ce = hipEventCreate(&b->ev[i]);
+  if (ce != hipSuccess) {
+    mpm_batch_destroy(b);
+    return hip_fail(ce, "mpm_batch_create");
+  }
+  *out = b;
+  return MPM_OK;
+}
+
+int mpm_batch_destroy(mpm_batch *b) {
+  if (!b) return MPM_OK;
+  (void)hipSetDevice(b->device);
+  while (!b->mem.empty()) (void)remove_member(b, b->mem.back());
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graphs(b);
+  if (b->dev) (void)hipFree(b->dev);
+  for (hipEvent_t &e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return MPM_OK;
+}
+
+int mpm_batch_add(mpm_batch *b, mpm_scene *s) {
+  if (!b || !s) return bfail(MPM_ERR_INVALID, "mpm_batch_add: NULL argument");
+  if (s->batch) return bfail(MPM_ERR_STATE, "scene already belongs to a batch");
+  const mpm_config &c = s->cfg;
+  if (c.device != b->device)
+    return bfail(MPM_ERR_INVALID, "scene on device %d, batch on device %d", c.device, b->device);
+  if (c.sort_period != 1 || (c.flags & MPM_FLAG_NO_LDS))
+    return bfail(MPM_ERR_UNSUPPORTED,
+                 "batched scenes re-sort every step on the LDS-staged path (sort_period 1)");
+  if (c.diffuse_capacity)
+    return bfail(MPM_ERR_UNSUPPORTED, "batched scenes carry no diffuse pool");
+  if (!b->mem.empty() && s->tc.margin != b->margin)
+    return bfail(MPM_ERR_INVALID, "tile_margin %u differs from the batch's %u", s->tc.margin,
+                 b->margin);
+  BHIP(hipSetDevice(b->device));
+  BHIP(hipStreamSynchronize(s->stream));
+  if (b->mem.empty()) b->margin = s->tc.margin;
+  s->own_stream = s->stream;
+  s->stream = b->stream;
+  s->batch = b;
+  b->mem.push_back(s);
+  b->dirty = true;
+  return MPM_OK;
+}
+
+int mpm_batch_remove(mpm_batch *b, mpm_scene *s) {
+  if (!b || !s) return bfail(MPM_ERR_INVALID, "mpm_batch_remove: NULL argument");
+  return remove_member(b, s);
+}
+
+int mpm_batch_count(const mpm_batch *b, uint32_t *count) {
+  if (!b || !count) return bfail(MPM_ERR_INVALID, "mpm_batch_count: NULL argument");
+  *count = (uint32_t)b->mem.size();
+  return MPM_OK;
+}
+
+int mpm_batch_set_tuning(mpm_batch *b, const mpm_batch_tuning *t) {
+  if (!b || !t) return bfail(MPM_ERR_INVALID, "mpm_batch_set_tuning: NULL argument");
+  if (t->graph_steps > 1024) return bfail(MPM_ERR_INVALID, "graph_steps %u > 1024", t->graph_steps);
+  BHIP(hipSetDevice(b->device));
+  BHIP(hipStreamSynchronize(b->stream));
+  b->tune = *t;
+  drop_graphs(b);
+  return MPM_OK;
+}
+
+int mpm_batch_get_tuning(const mpm_batch *b, mpm_batch_tuning *t) {
+  if (!b || !t) return bfail(MPM_ERR_INVALID, "mpm_batch_get_tuning: NULL argument");
+  *t = b->tune;
+  return MPM_OK;
+}
+
+int mpm_batch_step(mpm_batch *b, uint32_t nsteps) {
+  if (!b) return bfail(MPM_ERR_INVALID, "mpm_batch_step: batch is NULL");
+  if (b->mem.empty() || !nsteps) return MPM_OK;
+  BHIP(hipSetDevice(b->device));
+  // Scenes not yet sorted since their reset take their first step (the full
+  // sort of the caller's order) through the single-scene path, and so do the
+  // others for that step.
+  bool unsorted = false;
+  for (mpm_scene *s : b->mem) unsorted = unsorted || (s->n && !s->sorted_once);
+  if (unsorted) {
+    for (mpm_scene *s : b->mem) {
+      int rc = scene_step(s, 1);
+      if (rc) return rc;
+    }
+    if (--nsteps == 0) return MPM_OK;
+  }
+  if (!b->dirty)
+    for (size_t i = 0; i < b->mem.size(); i++)
+      if (b->mem[i]->version != b->seen[i]) b->dirty = true;
+  if (b->dirty) {
+    int rc = build_tables(b);
+    if (rc) return rc;
+  }
+  if (!b->tab[0].ns) {
+    advance(b, nsteps);
+    return MPM_OK;
+  }
+  const bool use_graph = !b->timing && !(b->tune.flags & MPM_BATCH_NO_GRAPH);
+  const uint32_t L = graph_len_of(b);
+  while (nsteps > 0) {
+    if (use_graph && nsteps >= L) {
+      if (!b->graph[b->phase] || b->graph_len != L) {
+        if (b->graph_len != L) drop_graphs(b);
+        int rc = capture(b, b->phase);
+        if (rc) return rc;
+      }
+      BHIP(hipGraphLaunch(b->graph[b->phase], b->stream));
+      advance(b, L);
+      nsteps -= L;
+      continue;
+    }
+    if (b->timing) {
+      if (b->ev_used == mpm_batch::kEv) {
+        int rc = fold_events(b);
+        if (rc) return rc;
+      }
+      BHIP(hipEventRecord(b->ev[2 * b->ev_used], b->stream));
+    }
+    launch_batch_step(b->tab[b->phase], b->margin, grid_of(b), b->stream);
+    BHIP(hipGetLastError());
+    if (b->timing) {
+      BHIP(hipEventRecord(b->ev[2 * b->ev_used + 1], b->stream));
+      b->ev_used++;
+    }
+    advance(b, 1);
+    nsteps--;
+  }
+  return MPM_OK;
+}
+
+int mpm_batch_sync(mpm_batch *b) {
+  if (!b) return bfail(MPM_ERR_INVALID, "mpm_batch_sync: batch is NULL");
+  BHIP(hipSetDevice(b->device));
+  BHIP(hipStreamSynchronize(b->stream));
+  return MPM_OK;
+}
+
+int mpm_batch_enable_timing(mpm_batch *b, int enable) {
+  if (!b) return bfail(MPM_ERR_INVALID, "mpm_batch_enable_timing: batch is NULL");
+  BHIP(hipSetDevice(b->device));
+  BHIP(hipStreamSynchronize(b->stream));
+  b->timing = enable != 0;
+  b->ev_used = 0;
+  b->step_ms.clear();
+  return MPM_OK;
+}
+
+int mpm_batch_step_times(mpm_batch *b, float *ms, uint32_t cap, uint32_t *count) {
+  if (!b) return bfail(MPM_ERR_INVALID, "mpm_batch_step_times: batch is NULL");
+  BHIP(hipSetDevice(b->device));
+  int rc = fold_events(b);
+  if (rc) return rc;
+  const uint32_t n = (uint32_t)b->step_ms.size();
+  if (count) *count = n;
+  if (ms)
+    for (uint32_t i = 0; i < n && i < cap; i++) ms[i] = b->step_ms[i];
+  return MPM_OK;
+}
+
+}  // extern "C"

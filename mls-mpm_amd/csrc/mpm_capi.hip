@@ -113,6 +113,7 @@ int mpm_create(const mpm_config *cfg, mpm_scene **out) {
 
 int mpm_destroy(mpm_scene *s) {
   if (!s) return MPM_OK;
+  (void)batch_detach(s);
   scene_free(s);
   delete s;
   return MPM_OK;
@@ -194,6 +195,7 @@ int mpm_set_params(mpm_scene *s, const mpm_config *cfg) {
   s->cfg = *cfg;
   s->cfg.seed = seed;
   scene_drop_graph(s);
+  s->version++;
   return MPM_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "../../include/mlsmpm.h"
+#include "mpm_batch.h"
 #include "mpm_kernels.h"
 #include "mpm_tiled.h"
 
@@ -63,6 +64,12 @@ struct mpm_scene {
   uint64_t graph_clock = 0;
   int graph_next = 0;            // timed: event half the next replay uses
   int graph_pending = -1;        // timed: event half not yet folded
+  // Batch membership: a member's work runs on the batch's stream.
+  mpm_batch *batch = nullptr;
+  hipStream_t own_stream = nullptr;
+  // Bumped by every change a batch's descriptor table depends on (steps
+  // outside the batch, reset, parameters, walls).
+  uint64_t version = 0;
   size_t bytes = 0;         // device bytes held by the scene
   void *owned[64] = {};     // allocations to free
   int nowned = 0;
@@ -90,4 +97,6 @@ void scene_drop_graph(mpm_scene *s);
 int scene_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count);
 Params scene_params(const mpm_scene *s);
 size_t scene_footprint(const mpm_config &c);
+void scene_dev(const mpm_scene *s, int phase, SceneDev *d);
+int batch_detach(mpm_scene *s);
 }  // namespace mpm

@@ -152,6 +152,24 @@ static PState pstate(const mpm_scene *s, int which) {
   return S;
 }
 
+// Descriptor of the scene for the batched kernels; phase 1 = the buffer
+// parity after one more re-sorting step.
+void scene_dev(const mpm_scene *s, int phase, SceneDev *d) {
+  memset((void *)d, 0, sizeof *d);
+  d->P = scene_params(s);
+  const int c = s->cur ^ phase;
+  d->S = pstate(s, c);
+  d->D = pstate(s, c ^ 1);
+  d->g = s->g;
+  d->bw = s->bw;
+  d->ko = key_out(s->sort, s->bw);
+  d->offsets = s->sort.offsets;
+  d->src_of = s->sort.src_of;
+  d->ch = 256u * s->tc.ppt;
+  d->keep = (s->cfg.flags & MPM_FLAG_KEEP_GRID) ? 1 : 0;
+  d->W = s->walls;
+}
+
 static int reset_diffuse(mpm_scene *s) {
   uint32_t D = s->cfg.diffuse_capacity;
   if (!D) return MPM_OK;
@@ -378,6 +396,7 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
   HIPCHK(hipSetDevice(c.device));
   HIPCHK(hipStreamSynchronize(s->stream));
   scene_drop_graph(s);
+  s->version++;
   s->n = n;
   s->cur = 0;
   s->steps = 0;
@@ -653,6 +672,7 @@ static int fold_graph_events(mpm_scene *s) {
 
 int scene_step(mpm_scene *s, uint32_t nsteps) {
   HIPCHK(hipSetDevice(s->cfg.device));
+  s->version++;
   const uint32_t period = s->cfg.sort_period;
   const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS) && period != 0;
   while (nsteps > 0) {
@@ -791,6 +811,7 @@ int scene_set_seed(mpm_scene *s, uint32_t seed) {
 
 int scene_set_walls(mpm_scene *s, const float *wallp) {
   scene_drop_graph(s);
+  s->version++;
   if (!wallp) {
     s->nwalls = 0;
     memset(&s->walls, 0, sizeof s->walls);

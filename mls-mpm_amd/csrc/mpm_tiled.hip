@@ -30,513 +30,57 @@
 // of the items of the surrounding bricks plus flagged overflow; the momentum
 // resolve also performs the grid update.  Integer addition is associative,
 // so every grid value is bit-identical to per-particle global atomics.
-#include "mpm_kernels.h"
-#include "mpm_ops.h"
-#include "mpm_tiled.h"
+#include "mpm_tiled_dev.h"
 
 namespace mpm {
 
-constexpr int kTB = 256;    // row width of an item (slots per row)
-constexpr int kMaxPpt = 16; // largest particles-per-thread of a work item
-constexpr int kCore = 10;   // brick + 1-cell halo per axis
-constexpr int kCoreV = kCore * kCore * kCore;
-constexpr uint32_t kFar = 31u;  // rank tag: particle left the 27-brick neighbourhood
-
-// Brick of a cell, clamped into the grid.
-__device__ __forceinline__ void cell_brick(const Params &P, float x, float y, float z, int &bx,
-                                           int &by, int &bz) {
-  bx = min(max((int)floorf(x), 0), (int)P.gx - 1) >> 3;
-  by = min(max((int)floorf(y), 0), (int)P.gy - 1) >> 3;
-  bz = min(max((int)floorf(z), 0), (int)P.gz - 1) >> 3;
-}
-
-__device__ __forceinline__ void brick_coords(const BrickWork &bw, uint32_t b, int &bx, int &by,
-                                             int &bz) {
-  bx = (int)(b % bw.nbx);
-  by = (int)((b / bw.nbx) % bw.nby);
-  bz = (int)(b / (bw.nbx * bw.nby));
-}
-
-__device__ __forceinline__ uint32_t brick_index(const BrickWork &bw, int bx, int by, int bz) {
-  return ((uint32_t)bz * bw.nby + (uint32_t)by) * bw.nbx + (uint32_t)bx;
-}
-
-// Global index of window cell c (edge E, origin o), or -1 outside the grid.
-template <int E>
-__device__ __forceinline__ int window_cell(const Params &P, int ox, int oy, int oz, int c) {
-  int wz = c / (E * E), rem = c - wz * (E * E);
-  int wy = rem / E, wx = rem - wy * E;
-  int x = ox + wx, y = oy + wy, z = oz + wz;
-  if (x < 0 || y < 0 || z < 0 || x >= (int)P.gx || y >= (int)P.gy || z >= (int)P.gz) return -1;
-  return (z * (int)P.gy + y) * (int)P.gx + x;
-}
-
-__device__ __forceinline__ void flag_cell_brick(const BrickWork &bw, int x, int y, int z) {
-  bw.ofl[brick_index(bw, x >> 3, y >> 3, z >> 3)] = 1;
-}
-
-// A particle whose stencil leaves the window core contributes to margin
-// cells (or, past the margin, straight to the overflow grid): flag the
-// bricks of its stencil corners so the resolve folds the overflow in.
-__device__ __forceinline__ void flag_stencil(const Params &P, const BrickWork &bw, const PW &pw) {
-#pragma unroll
-  for (int c = 0; c < 8; c++) {
-    int x = min(max(pw.b[0] + ((c & 1) ? 2 : 0), 0), (int)P.gx - 1);
-    int y = min(max(pw.b[1] + ((c & 2) ? 2 : 0), 0), (int)P.gy - 1);
-    int z = min(max(pw.b[2] + ((c & 4) ? 2 : 0), 0), (int)P.gz - 1);
-    flag_cell_brick(bw, x, y, z);
-  }
-}
-
-// Window position of a particle's base cell and whether its stencil lies in
-// the window / in the window core.
-struct WinPos {
-  int lb;
-  bool inside, in_core;
-};
-template <int M>
-__device__ __forceinline__ WinPos win_pos(const PW &pw, int ox, int oy, int oz) {
-  constexpr int E = 10 + 2 * M;
-  int lx = pw.b[0] - ox, ly = pw.b[1] - oy, lz = pw.b[2] - oz;
-  WinPos w;
-  w.inside = lx >= 0 && ly >= 0 && lz >= 0 && lx + 2 < E && ly + 2 < E && lz + 2 < E;
-  w.in_core = lx >= M && ly >= M && lz >= M && lx + 2 < M + kCore && ly + 2 < M + kCore &&
-              lz + 2 < M + kCore;
-  w.lb = (lz * E + ly) * E + lx;
-  return w;
-}
-
 // ---------------------------------------------------------------------------
-// P2G mass over brick work items (WGSL_P2G_MASS, shaders.js:242-274).  With
-// ko.counts set, also assigns every particle its brick and rank for the
-// re-sort.
+// Kernels: one block / thread per index of the device bodies.
 template <int M, bool UNIT>
 __global__ __launch_bounds__(256) void k_p2g_mass_brick(Params P, PState S, BrickWork bw,
                                                         KeyOut ko) {
-  constexpr int E = 10 + 2 * M, V = E * E * E;
-  __shared__ int32_t lds[V];
-  __shared__ uint32_t lcnt[27], lbase[27];
-  if (blockIdx.x >= *bw.nitems) return;
-  const uint4 it = bw.items[blockIdx.x];
-  int ibx, iby, ibz;
-  brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
-  const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
-  const int t = threadIdx.x, nt = blockDim.x;
-  for (int c = t; c < V; c += nt) lds[c] = 0;
-  if (t < 27) lcnt[t] = 0;
-  __syncthreads();
-  for (uint32_t idx = t; idx < it.z; idx += nt) {
-    uint32_t s = it.y + idx;
-    float px = S.at(PX, s), py = S.at(PY, s), pz = S.at(PZ, s);
-    PW pw;
-    particle_weights(px, py, pz, pw);
-    if (ko.counts) {
-      int bx, by, bz;
-      cell_brick(P, px, py, pz, bx, by, bz);
-      int dx = bx - ibx + 1, dy = by - iby + 1, dz = bz - ibz + 1;
-      uint32_t dest = bw.brick_rank[brick_index(bw, bx, by, bz)];
-      ko.keys[s] = dest;
-      if (dx >= 0 && dx < 3 && dy >= 0 && dy < 3 && dz >= 0 && dz < 3) {
-        uint32_t nb = (uint32_t)((dz * 3 + dy) * 3 + dx);
-        ko.rank[s] = (nb << 26) | atomicAdd(&lcnt[nb], 1u);
-      } else {
-        ko.rank[s] = (kFar << 26) | atomicAdd(&ko.counts[dest], 1u);
-      }
-    }
-    float pm = UNIT ? 1.0f : S.at(PM, s);
-    WinPos wp = win_pos<M>(pw, ox, oy, oz);
-    if (!wp.in_core) flag_stencil(P, bw, pw);
-    if (wp.inside) {
-#pragma unroll
-      for (int kk = 0; kk < 3; kk++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          float wxy[3];
-#pragma unroll
-          for (int i = 0; i < 3; i++) wxy[i] = pw.w[0][i] * pw.w[1][j];
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            float w = wxy[i] * pw.w[2][kk];
-            atomicAdd(&lds[wp.lb + (kk * E + j) * E + i], encode(UNIT ? w : w * pm));
-          }
-        }
-    } else {  // beyond the window: overflow atomics
-      for (int kk = 0; kk < 3; kk++)
-        for (int j = 0; j < 3; j++)
-          for (int i = 0; i < 3; i++) {
-            int gi = cell_idx_checked(pw.b[0] + i, pw.b[1] + j, pw.b[2] + kk, P);
-            if (gi < 0) continue;
-            float w = (pw.w[0][i] * pw.w[1][j]) * pw.w[2][kk];
-            atomicAdd(&bw.om[gi], encode(UNIT ? w : w * pm));
-          }
-    }
-  }
-  __syncthreads();
-  if (ko.counts) {
-    if (t < 27) {
-      uint32_t c = lcnt[t], base = 0;
-      if (c) {
-        int bx = ibx + t % 3 - 1, by = iby + (t / 3) % 3 - 1, bz = ibz + t / 9 - 1;
-        base = atomicAdd(&ko.counts[bw.brick_rank[brick_index(bw, bx, by, bz)]], c);
-      }
-      lbase[t] = base;
-    }
-    __syncthreads();
-    for (uint32_t idx = t; idx < it.z; idx += nt) {
-      uint32_t s = it.y + idx;
-      uint32_t r = ko.rank[s], nb = r >> 26;
-      if (nb != kFar) ko.rank[s] = lbase[nb] + (r & 0x3ffffffu);
-      else ko.rank[s] = r & 0x3ffffffu;
-    }
-  }
-  // core to the ghost buffer, non-zero margin cells to the overflow grid
-  int32_t *gh = bw.gm + (size_t)blockIdx.x * kCoreV;
-  for (int c = t; c < kCoreV; c += nt) {
-    int cz = c / 100, cy = (c / 10) % 10, cx = c % 10;
-    gh[c] = lds[((cz + M) * E + (cy + M)) * E + (cx + M)];
-  }
-  if (M > 0) {
-    for (int c = t; c < V; c += nt) {
-      int32_t v = lds[c];
-      if (v == 0) continue;
-      int wz = c / (E * E), wy = (c / E) % E, wx = c % E;
-      bool core = wx >= M && wy >= M && wz >= M && wx < M + kCore && wy < M + kCore &&
-                  wz < M + kCore;
-      if (core) continue;
-      int gi = window_cell<E>(P, ox, oy, oz, c);
-      if (gi >= 0) atomicAdd(&bw.om[gi], v);
-    }
-  }
+  p2g_mass_item<M, UNIT>(P, S, bw, ko, blockIdx.x);
 }
 
-// ---------------------------------------------------------------------------
-// P2G momentum over brick work items (WGSL_P2G_MOM, shaders.js:286-370): the
-// decoded mass window and the momentum window live in LDS.
 template <int M, bool UNIT>
 __global__ __launch_bounds__(256) void k_p2g_mom_brick(Params P, PState S, GState g,
                                                        BrickWork bw) {
-  constexpr int E = 10 + 2 * M, V = E * E * E;
-  __shared__ float lm[V];
-  __shared__ int32_t lv[3 * V];
-  if (blockIdx.x >= *bw.nitems) return;
-  const uint4 it = bw.items[blockIdx.x];
-  int ibx, iby, ibz;
-  brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
-  const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
-  const int t = threadIdx.x, nt = blockDim.x;
-  for (int c = t; c < V; c += nt) {
-    int gi = window_cell<E>(P, ox, oy, oz, c);
-    lm[c] = gi >= 0 ? decode(g.m[gi]) : 0.0f;
-    lv[3 * c] = 0;
-    lv[3 * c + 1] = 0;
-    lv[3 * c + 2] = 0;
-  }
-  __syncthreads();
-  const uint32_t gx = P.gx, slice = P.gx * P.gy;
-  // Software pipeline: the next particle's inputs load while this one computes.
-  // The prefetch index past the item's end is clamped to its last particle
-  // (in bounds, unconditional).
-  uint32_t idx = t;
-  MomIn cur;
-  if (idx < it.z) load_mom_in<UNIT>(S, it.y + idx, cur);
-  while (idx < it.z) {
-    const uint32_t s = it.y + idx, nidx = idx + nt;
-    MomIn nxt;
-    load_mom_in<UNIT>(S, it.y + (nidx < it.z ? nidx : it.z - 1), nxt);
-    PW pw;
-    particle_weights(cur.px, cur.py, cur.pz, pw);
-    WinPos wp = win_pos<M>(pw, ox, oy, oz);
-    float density;
-    if (wp.inside) {
-      const int lb = wp.lb;
-      density = p2g_mom_body<UNIT>(
-          P, cur, pw, [&](int kk, int j, int i) { return lm[lb + (kk * E + j) * E + i]; },
-          [&](int kk, int j, int i, int32_t ex, int32_t ey, int32_t ez) {
-            int c = lb + (kk * E + j) * E + i;
-            atomicAdd(&lv[3 * c], ex);
-            atomicAdd(&lv[3 * c + 1], ey);
-            atomicAdd(&lv[3 * c + 2], ez);
-          });
-    } else {  // beyond the window: global gather, overflow atomics
-      uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
-      density = p2g_mom_body<UNIT>(
-          P, cur, pw,
-          [&](int kk, int j, int i) { return decode(g.m[base + kk * slice + j * gx + i]); },
-          [&](int kk, int j, int i, int32_t ex, int32_t ey, int32_t ez) {
-            uint32_t c = base + kk * slice + j * gx + i;
-            atomicAdd(&bw.ox[c], ex);
-            atomicAdd(&bw.oy[c], ey);
-            atomicAdd(&bw.oz[c], ez);
-          });
-    }
-    S.density[s] = density;
-    cur = nxt;
-    idx = nidx;
-  }
-  __syncthreads();
-  const size_t gb = (size_t)blockIdx.x * kCoreV;
-  for (int c = t; c < kCoreV; c += nt) {
-    int cz = c / 100, cy = (c / 10) % 10, cx = c % 10;
-    int w = ((cz + M) * E + (cy + M)) * E + (cx + M);
-    bw.gx[gb + c] = lv[3 * w];
-    bw.gy[gb + c] = lv[3 * w + 1];
-    bw.gz[gb + c] = lv[3 * w + 2];
-  }
-  if (M > 0) {
-    for (int c = t; c < V; c += nt) {
-      int32_t ex = lv[3 * c], ey = lv[3 * c + 1], ez = lv[3 * c + 2];
-      if ((ex | ey | ez) == 0) continue;
-      int wz = c / (E * E), wy = (c / E) % E, wx = c % E;
-      bool core = wx >= M && wy >= M && wz >= M && wx < M + kCore && wy < M + kCore &&
-                  wz < M + kCore;
-      if (core) continue;
-      int gi = window_cell<E>(P, ox, oy, oz, c);
-      if (gi < 0) continue;
-      if (ex) atomicAdd(&bw.ox[gi], ex);
-      if (ey) atomicAdd(&bw.oy[gi], ey);
-      if (ez) atomicAdd(&bw.oz[gi], ez);
-    }
-  }
+  p2g_mom_item<M, UNIT>(P, S, g, bw, blockIdx.x);
 }
 
-// ---------------------------------------------------------------------------
-// Resolve.  list holds the bricks whose cells can be non-zero this step or
-// were non-zero last step: bricks next to an occupied brick, flagged bricks
-// and bricks active last step.  One block (512 threads, one per cell) per
-// listed brick, persistent over the list.
 __global__ __launch_bounds__(256) void k_brick_list(BrickWork bw) {
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= bw.nbricks) return;
-  int bx, by, bz;
-  brick_coords(bw, b, bx, by, bz);
-  bool need = (bw.act[b] & 1) || bw.ofl[b];
-  for (int n = 0; n < 27 && !need; n++) {
-    int x = bx + n % 3 - 1, y = by + (n / 3) % 3 - 1, z = bz + n / 9 - 1;
-    if (x < 0 || y < 0 || z < 0 || x >= (int)bw.nbx || y >= (int)bw.nby || z >= (int)bw.nbz)
-      continue;
-    need = bw.occ[brick_index(bw, x, y, z)] != 0;
-  }
-  if (need) bw.list[atomicAdd(bw.nlist, 1u)] = b;
+  brick_list_one(bw, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-struct NbrItems {
-  uint32_t first[27], count[27];
-};
-
-__device__ __forceinline__ bool load_neighbours(const BrickWork &bw, uint32_t b, NbrItems &nb,
-                                                int t) {
-  __shared__ int any;
-  if (t == 0) any = 0;
-  __syncthreads();
-  if (t < 27) {
-    int bx, by, bz;
-    brick_coords(bw, b, bx, by, bz);
-    bx += t % 3 - 1;
-    by += (t / 3) % 3 - 1;
-    bz += t / 9 - 1;
-    uint32_t cnt = 0, first = 0;
-    if (bx >= 0 && by >= 0 && bz >= 0 && bx < (int)bw.nbx && by < (int)bw.nby &&
-        bz < (int)bw.nbz) {
-      uint32_t r = bw.brick_rank[brick_index(bw, bx, by, bz)];
-      cnt = bw.count[r];
-      first = bw.first[r];
-    }
-    nb.first[t] = first;
-    nb.count[t] = cnt;
-    if (cnt) atomicOr(&any, 1);
-  }
-  __syncthreads();
-  return any != 0;
-}
-
-// Calls f(ghost index) for every ghost core that covers cell (lx,ly,lz) of
-// the brick (local coords 0..7): its own brick's items, and the neighbour
-// bricks' items whose core reaches one cell into this brick.
-template <class F>
-__device__ __forceinline__ void for_covering_items(const NbrItems &nb, int lx, int ly, int lz,
-                                                   F f) {
-  const int x0 = lx == 0 ? 0 : 1, x1 = lx == 7 ? 2 : 1;
-  const int y0 = ly == 0 ? 0 : 1, y1 = ly == 7 ? 2 : 1;
-  const int z0 = lz == 0 ? 0 : 1, z1 = lz == 7 ? 2 : 1;
-  for (int dz = z0; dz <= z1; dz++)
-    for (int dy = y0; dy <= y1; dy++)
-      for (int dx = x0; dx <= x1; dx++) {
-        int n = (dz * 3 + dy) * 3 + dx;
-        if (!nb.count[n]) continue;
-        // core coordinates of this cell in brick n (core origin = brick n - 1)
-        int cx = lx - (dx - 1) * 8 + 1, cy = ly - (dy - 1) * 8 + 1, cz = lz - (dz - 1) * 8 + 1;
-        int li = (cz * kCore + cy) * kCore + cx;
-        for (uint32_t k = 0; k < nb.count[n]; k++) f((size_t)(nb.first[n] + k) * kCoreV + li);
-      }
-}
-
-__global__ __launch_bounds__(512) void k_resolve_mass(Params P, GState g, BrickWork bw) {
-  __shared__ NbrItems nb;
-  const int t = threadIdx.x;
-  const uint32_t nl = *bw.nlist;
-  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) {
-    const uint32_t b = bw.list[i];
-    bool any = load_neighbours(bw, b, nb, t);
-    const bool ofl = bw.ofl[b] != 0;
-    int bx, by, bz;
-    brick_coords(bw, b, bx, by, bz);
-    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
-    const int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
-    if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
-      uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
-      uint32_t sum = 0;
-      if (any) for_covering_items(nb, lx, ly, lz, [&](size_t gi) { sum += (uint32_t)bw.gm[gi]; });
-      if (ofl) {
-        sum += (uint32_t)bw.om[cell];
-        bw.om[cell] = 0;
-      }
-      g.m[cell] = (int32_t)sum;
-    }
-    if (t == 0) {
-      uint8_t a = bw.act[b];
-      bw.act[b] = (uint8_t)(((a & 1) << 1) | ((any || ofl) ? 1 : 0));
-    }
-    __syncthreads();
-  }
-}
-
-// Momentum resolve fused with the grid update (WGSL_UPDATE_GRID,
-// shaders.js:443-512): (vx, vy, vz, decoded mass) per cell.
-__global__ __launch_bounds__(512) void k_resolve_update(Params P, GState g, BrickWork bw, Walls W,
-                                                        int keep) {
-  __shared__ NbrItems nb;
-  const int t = threadIdx.x;
-  const uint32_t nl = *bw.nlist;
-  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) {
-    const uint32_t b = bw.list[i];
-    bool any = load_neighbours(bw, b, nb, t);
-    const bool ofl = bw.ofl[b] != 0;
-    int bx, by, bz;
-    brick_coords(bw, b, bx, by, bz);
-    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
-    const int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
-    if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
-      uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
-      uint32_t sx = 0, sy = 0, sz = 0;
-      if (any)
-        for_covering_items(nb, lx, ly, lz, [&](size_t gi) {
-          sx += (uint32_t)bw.gx[gi];
-          sy += (uint32_t)bw.gy[gi];
-          sz += (uint32_t)bw.gz[gi];
-        });
-      if (ofl) {
-        sx += (uint32_t)bw.ox[cell];
-        sy += (uint32_t)bw.oy[cell];
-        sz += (uint32_t)bw.oz[cell];
-        bw.ox[cell] = 0;
-        bw.oy[cell] = 0;
-        bw.oz[cell] = 0;
-      }
-      g.v[cell] = update_cell(P, W, g.m[cell], (int32_t)sx, (int32_t)sy, (int32_t)sz, x, y, z);
-      if (keep) {
-        g.mx[cell] = (int32_t)sx;
-        g.my[cell] = (int32_t)sy;
-        g.mz[cell] = (int32_t)sz;
-      }
-    }
-    if (t == 0) bw.ofl[b] = 0;
-    __syncthreads();
-  }
-}
-
-// ---------------------------------------------------------------------------
-// G2P over brick work items (WGSL_G2P, shaders.js:589-704).  With src_of set,
-// slot d of the destination state takes the particle at old slot src_of[d]
-// (re-sort; mass, id and density move along) and every particle lies in the
-// item's brick (EXACT: no particle can leave the window).  Otherwise G2P runs
-// in place over the current items with the drift margin.
 template <int M, bool EXACT>
 __global__ __launch_bounds__(256) void k_g2p_brick(Params P, PState S, PState D, GState g,
                                                    Walls W, BrickWork bw,
                                                    const uint32_t *src_of) {
-  constexpr int E = 10 + 2 * M, V = E * E * E;
-  __shared__ float4 lv[V];
-  // Re-sort: the item's source slots, staged with the window.
-  __shared__ uint32_t lsrc[EXACT ? kTB * kMaxPpt : 1];
-  if (blockIdx.x >= *bw.nitems) return;
-  const uint4 it = bw.items[blockIdx.x];
-  int ibx, iby, ibz;
-  brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
-  const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
-  const int t = threadIdx.x, nt = blockDim.x;
-  for (int c = t; c < V; c += nt) {
-    int gi = window_cell<E>(P, ox, oy, oz, c);
-    lv[c] = gi >= 0 ? g.v[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  }
-  if (EXACT)
-    for (uint32_t i = t; i < it.z; i += nt) lsrc[i] = src_of[it.y + i];
-  __syncthreads();
-  uint32_t idx = t;
-  if (idx >= it.z) return;
-  const uint32_t gx = P.gx, slice = P.gx * P.gy;
-  // Software pipeline: the next particle's inputs (for a re-sort also the
-  // mass, id and density it moves) load while this one computes.  Prefetch
-  // indexes past the item's end are clamped to its last particle, so every
-  // load is in bounds and unconditional.  The first particle is peeled off
-  // the loop: vmcnt counts loads and stores in issue order, and with both
-  // paths into the loop alike (loads, then the previous particle's stores)
-  // the wait for a particle's inputs does not cover the stores before it.
-  struct In {
-    uint32_t s, id;
-    float px, py, pz, m, dens;
-  };
-  auto load_in = [&](uint32_t i, In &in) {
-    const uint32_t j = i < it.z ? i : it.z - 1;
-    in.s = EXACT ? lsrc[j] : it.y + j;
-    in.px = S.at(PX, in.s);
-    in.py = S.at(PY, in.s);
-    in.pz = S.at(PZ, in.s);
-    if (EXACT) {
-      in.m = S.at(PM, in.s);
-      in.id = S.id[in.s];
-      in.dens = S.density[in.s];
-    }
-  };
-  auto body = [&](uint32_t i, const In &cur) {
-    const uint32_t d = it.y + i;
-    const float px = cur.px, py = cur.py, pz = cur.pz;
-    PW pw;
-    particle_weights(px, py, pz, pw);
-    WinPos wp = win_pos<M>(pw, ox, oy, oz);
-    G2POut o;
-    if (EXACT || wp.inside) {
-      const int lb = wp.lb;
-      g2p_particle(P, W, px, py, pz, pw, [&](int kk, int j, int ii) {
-        float4 v = lv[lb + (kk * E + j) * E + ii];
-        return make_float3(v.x, v.y, v.z);
-      }, o);
-    } else {
-      uint32_t base = (uint32_t)((pw.b[2] * (int)P.gy + pw.b[1]) * (int)gx + pw.b[0]);
-      g2p_particle(P, W, px, py, pz, pw, [&](int kk, int j, int ii) {
-        float4 v = g.v[base + kk * slice + j * gx + ii];
-        return make_float3(v.x, v.y, v.z);
-      }, o);
-    }
-    store_g2p(D, d, o);
-    if (EXACT) {
-      D.at(PM, d) = cur.m;
-      D.id[d] = cur.id;
-      D.density[d] = cur.dens;
-    }
-  };
-  In cur, nxt;
-  load_in(idx, cur);
-  load_in(idx + nt, nxt);
-  body(idx, cur);
-  for (idx += nt; idx < it.z; idx += nt) {
-    cur = nxt;
-    load_in(idx + nt, nxt);
-    body(idx, cur);
-  }
+  g2p_item<M, EXACT>(P, S, D, g, W, bw, src_of, blockIdx.x);
 }
 
-// ---------------------------------------------------------------------------
-// Sort plumbing.
+__global__ __launch_bounds__(256) void k_make_items(uint32_t n, BrickWork bw, uint32_t ch,
+                                                    const uint32_t *offsets) {
+  make_items_one(n, bw, ch, offsets, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_build_src(uint32_t n, uint32_t nbricks, uint32_t ch,
+                                                   const uint32_t *keys, const uint32_t *rank,
+                                                   const uint32_t *offsets, uint32_t *src_of) {
+  build_src_one(n, nbricks, ch, keys, rank, offsets, src_of, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(512) void k_resolve_mass(Params P, GState g, BrickWork bw) {
+  const uint32_t nl = *bw.nlist;
+  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) resolve_mass_entry(P, g, bw, i);
+}
+
+__global__ __launch_bounds__(512) void k_resolve_update(Params P, GState g, BrickWork bw, Walls W,
+                                                        int keep) {
+  const uint32_t nl = *bw.nlist;
+  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) resolve_update_entry(P, g, bw, W, keep, i);
+}
+
 // First-step keys: brick and rank of every particle by global atomics.
 __global__ __launch_bounds__(256) void k_sort_keys(Params P, PState S, BrickWork bw, KeyOut ko) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -548,69 +92,7 @@ __global__ __launch_bounds__(256) void k_sort_keys(Params P, PState S, BrickWork
   ko.rank[s] = atomicAdd(&ko.counts[dest], 1u);
 }
 
-// Work items from the brick offsets: each brick's particle range cut into
-// pieces of at most `ch` particles.  Item order is irrelevant to the result.
-__global__ __launch_bounds__(256) void k_make_items(uint32_t n, BrickWork bw, uint32_t ch,
-                                                    const uint32_t *offsets) {
-  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= bw.nbricks) return;
-  uint32_t start = offsets[r];
-  uint32_t end = (r + 1 < bw.nbricks) ? offsets[r + 1] : n;
-  uint32_t b = bw.brick_of_rank[r];
-  if (end <= start) {
-    bw.count[r] = 0;
-    bw.first[r] = 0;
-    bw.occ[b] = 0;
-    return;
-  }
-  uint32_t cnt = end - start;
-  uint32_t pieces = (cnt + ch - 1) / ch;
-  uint32_t at = atomicAdd(bw.nitems, pieces);
-  bw.first[r] = at;
-  bw.count[r] = pieces;
-  bw.occ[b] = 1;
-  for (uint32_t p = 0; p < pieces; p++) {
-    uint32_t c = cnt - p * ch;
-    bw.items_w[at + p] = make_uint4(r, start + p * ch, c < ch ? c : ch, 0u);
-  }
-}
-
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
-
-// Slot of the particle of rank r among the c particles of an item: ranks are
-// stored column-major over rows of 256 slots.  With K = ceil(c/256), the
-// first L = c - 256(K-1) columns hold K ranks, the others K-1.
-__device__ __forceinline__ uint32_t item_slot(uint32_t r, uint32_t c) {
-  uint32_t K = (c + 255u) / 256u;
-  uint32_t L = c - 256u * (K - 1u);
-  uint32_t t, k;
-  if (r < L * K) {
-    t = r / K;
-    k = r - t * K;
-  } else {
-    uint32_t r2 = r - L * K;
-    t = L + r2 / (K - 1u);
-    k = r2 % (K - 1u);
-  }
-  return k * 256u + t;
-}
-
-// Inverse permutation of the re-sort: src_of[new slot] = old slot.
-__global__ __launch_bounds__(256) void k_build_src(uint32_t n, uint32_t nbricks, uint32_t ch,
-                                                   const uint32_t *keys, const uint32_t *rank,
-                                                   const uint32_t *offsets, uint32_t *src_of) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  uint32_t brick = keys[s];
-  uint32_t bstart = offsets[brick];
-  uint32_t bend = (brick + 1 < nbricks) ? offsets[brick + 1] : n;
-  uint32_t local = rank[s];
-  uint32_t piece = local / ch;
-  uint32_t istart = bstart + piece * ch;
-  uint32_t c = bend - istart;
-  if (c > ch) c = ch;
-  src_of[istart + item_slot(local - piece * ch, c)] = s;
-}
 
 // Gather form of the first step's data move: slot d takes old slot src_of[d].
 __global__ __launch_bounds__(256) void k_sort_gather(uint32_t n, PState src, PState dst,

@@ -1,14 +1,17 @@
 """mlsmpm -- MI355X (gfx950) MLS-MPM fluid step behind a C ABI.
 
 Python side of include/mlsmpm.h: ctypes binding (capi), scene handle
-(scene.Scene), seeded scene layouts (scenes) and the build recipe (build).
+(scene.Scene), batches of scenes (batch.Batch), seeded scene layouts
+(scenes) and the build recipe (build).
 The compute path is libmlsmpm.so only.
 """
 from .capi import (Config, MpmError, MPM_FLAG_NO_GRAPH, MPM_FLAG_KEEP_GRID, MPM_FLAG_NO_LDS,
                    config_from_aspect, default_config, load, lib_path)
 from .scene import DREC_DTYPE, Scene
+from .batch import Batch
+from .capi import BatchTuning, MPM_BATCH_NO_GRAPH
 
 __all__ = [
-    "Config", "MpmError", "Scene", "DREC_DTYPE", "default_config", "config_from_aspect", "load",
+    "Config", "MpmError", "Scene", "Batch", "BatchTuning", "MPM_BATCH_NO_GRAPH", "DREC_DTYPE", "default_config", "config_from_aspect", "load",
     "lib_path", "MPM_FLAG_KEEP_GRID", "MPM_FLAG_NO_LDS", "MPM_FLAG_NO_GRAPH",
 ]

@@ -65,6 +65,17 @@ class Config(C.Structure):
         return c
 
 
+MPM_BATCH_NO_GRAPH = 1
+
+
+class BatchTuning(C.Structure):
+    """mpm_batch_tuning: persistent blocks per batched kernel class (0 = one
+    block per unit), steps per captured graph, flags."""
+    _fields_ = [("item_blocks", C.c_uint32), ("resolve_blocks", C.c_uint32),
+                ("aux_blocks", C.c_uint32), ("graph_steps", C.c_uint32), ("flags", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
+
+
 # Every entry point declared in include/mlsmpm.h, with its argument types.
 _VP = C.c_void_p
 _SIGNATURES = {
@@ -92,6 +103,17 @@ _SIGNATURES = {
     "mpm_stage_times": ([_VP, _VP], C.c_int),
     "mpm_step_times": ([_VP, _VP, C.c_uint32, _VP], C.c_int),
     "mpm_tile_stats": ([_VP, _VP], C.c_int),
+    "mpm_batch_create": ([C.c_int, _VP], C.c_int),
+    "mpm_batch_destroy": ([_VP], C.c_int),
+    "mpm_batch_add": ([_VP, _VP], C.c_int),
+    "mpm_batch_remove": ([_VP, _VP], C.c_int),
+    "mpm_batch_count": ([_VP, _VP], C.c_int),
+    "mpm_batch_set_tuning": ([_VP, _VP], C.c_int),
+    "mpm_batch_get_tuning": ([_VP, _VP], C.c_int),
+    "mpm_batch_step": ([_VP, C.c_uint32], C.c_int),
+    "mpm_batch_sync": ([_VP], C.c_int),
+    "mpm_batch_enable_timing": ([_VP, C.c_int], C.c_int),
+    "mpm_batch_step_times": ([_VP, _VP, C.c_uint32, _VP], C.c_int),
 }
 
 _lib = None
