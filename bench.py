@@ -109,6 +109,141 @@ def cpu_baseline(grid, pos, vel, seconds: float) -> dict:
     }
 
 
+def batch_bench(args) -> dict:
+    """Batched form against sequential plain steps of the same scene mix:
+    `batch_scenes` dam-break scenes (grid edges log-uniform 32..128 per axis,
+    10k..2M particles log-uniform, capped at 8 per cell over half the
+    interior), every scene re-sorting every step.  Batched: all scenes in one
+    mpm_batch (one persistent launch per stage, captured graphs, launch shapes
+    from the autotuner).  Sequential: each scene stepped alone (its own
+    captured graphs), one scene after another.  Rates are particle-steps/s
+    over the same number of steps."""
+    import mlsmpm
+    from mlsmpm import autotune, scenes
+
+    mix = scenes.scene_mix(args.seed + 3, args.batch_scenes)
+    ntot = sum(len(p) for _, p, _, _ in mix)
+    print(f"[bench] batch: {len(mix)} scenes, {ntot} particles", file=sys.stderr, flush=True)
+    sc = []
+    for grid, pos, vel, _ in mix:
+        s = mlsmpm.Scene(mlsmpm.default_config(*grid, len(pos), sort_period=1, device=0))
+        s.reset(pos, vel)
+        sc.append(s)
+    K, W = args.batch_steps, 17
+    b = mlsmpm.Batch()
+    for s in sc:
+        b.add(s)
+    tune = autotune.tune_batch(b)
+    b.step(W)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(K)
+    b.sync()
+    tb = time.perf_counter() - t0
+    b.enable_timing(True)
+    b.step(min(K, 32))
+    lat = b.step_times()
+    b.close()
+    for s in sc:
+        s.step(W)
+    for s in sc:
+        s.sync()
+    t0 = time.perf_counter()
+    for s in sc:
+        s.step(K)
+        s.sync()
+    tseq = time.perf_counter() - t0
+    for s in sc:
+        s.close()
+    batched = ntot * K / tb
+    seq = ntot * K / tseq
+    return {
+        "scenes": len(mix),
+        "particles": ntot,
+        "steps": K,
+        "value": batched,
+        "unit": "particle-steps/s",
+        "ms_per_step": tb / K * 1e3,
+        "sequential_plain": seq,
+        "sequential_ms_per_step": tseq / K * 1e3,
+        "vs_sequential": batched / seq,
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50)),
+                            "p99": float(np.percentile(lat, 99))},
+        "tuning": {"item_blocks": tune.item_blocks, "resolve_blocks": tune.resolve_blocks,
+                   "aux_blocks": tune.aux_blocks, "graph_steps": tune.graph_steps},
+    }
+
+
+def serve_bench(args) -> dict:
+    """Serving form: a seeded Poisson stream of requests (dam-break or
+    two-block scene, grid edges 32..128, 10k..2M particles log-uniform, steps
+    uniform in 100..2000) at `serve_rate` per second for `serve_seconds`,
+    admitted by an HBM budget of 200 GB into one continuously batched loop.
+    Reports aggregate particle-steps/s (over the loop's busy time and over the
+    wall time from first arrival to last completion), p50/p99 batched-step
+    latency and p50/p99 request latency; `vs_plain` divides the busy-time rate
+    by the same requests' plain single-scene rate (each scene's per-step time
+    measured alone, times its step count, summed)."""
+    import mlsmpm
+    from mlsmpm import scenes
+
+    rng = np.random.default_rng(args.seed + 11)
+    nreq = max(1, int(round(args.serve_rate * args.serve_seconds)))
+    mix = scenes.scene_mix(args.seed + 5, nreq, kinds=("dam", "two"))
+    steps = rng.integers(100, 2001, nreq)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, nreq))
+    work = sum(len(p) * int(k) for (_, p, _, _), k in zip(mix, steps))
+    print(f"[bench] serve: {nreq} requests, {work:.3e} particle-steps", file=sys.stderr,
+          flush=True)
+    srv = mlsmpm.Server(hbm_budget=200 << 30, chunk_steps=args.serve_chunk)
+    ids = []
+    t0 = time.perf_counter()
+    for (grid, pos, vel, _), k, ta in zip(mix, steps, arrivals):
+        dt = ta - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(mlsmpm.default_config(*grid, len(pos)), pos, int(k), vel=vel))
+    lat = []
+    for rid in ids:
+        srv.wait(rid, 600.0)
+        info = srv.poll(rid)
+        lat.append(info["t_done"] - info["t_submit"])
+        srv.result(rid)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    step_ms = srv.step_times()
+    srv.close()
+    # plain form: every scene alone, per-step time over 32 graph-replayed steps
+    plain_s = 0.0
+    for (grid, pos, vel, _), k in zip(mix, steps):
+        with mlsmpm.Scene(mlsmpm.default_config(*grid, len(pos), sort_period=1)) as s:
+            s.reset(pos, vel)
+            s.step(17)
+            s.sync()
+            ta = time.perf_counter()
+            s.step(32)
+            s.sync()
+            plain_s += (time.perf_counter() - ta) / 32 * int(k)
+    busy_rate = st["particle_steps"] / st["busy_seconds"]
+    return {
+        "requests": nreq,
+        "completed": st["completed"],
+        "rate_per_s": args.serve_rate,
+        "hbm_budget_gb": 200,
+        "particle_steps": st["particle_steps"],
+        "value": busy_rate,
+        "unit": "particle-steps/s",
+        "wall_rate": st["particle_steps"] / wall,
+        "plain_rate": work / plain_s,
+        "vs_plain": busy_rate / (work / plain_s),
+        "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                            "p99": float(np.percentile(step_ms, 99))},
+        "request_latency_s": {"p50": float(np.percentile(lat, 50)),
+                              "p99": float(np.percentile(lat, 99))},
+        "batched_steps": st["batched_steps"],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +261,13 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-events", action="store_true",
                     help="skip the event-timed region (no stage times, no roofline)")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=48)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=12.0)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-chunk", type=int, default=8)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES each (the plain copy that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -245,6 +387,10 @@ def main() -> int:
         },
     }
     s.close()
+    if not args.no_batch:
+        line["batched"] = batch_bench(args)
+    if not args.no_serve:
+        line["served"] = serve_bench(args)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(grid, pos, vel, args.cpu_seconds)

@@ -226,6 +226,78 @@ int mpm_batch_sync(mpm_batch *b);
 int mpm_batch_enable_timing(mpm_batch *b, int enable);
 int mpm_batch_step_times(mpm_batch *b, float *ms, uint32_t cap, uint32_t *count);
 
+/* ------------------------------------------------------------------------
+ * Serving form.  No counterpart in the reference: a request loop with
+ * continuous batching.  Requests (scene configuration, particles, step count)
+ * are queued from any thread; one loop thread owns the device, admits queued
+ * requests in arrival order while the HBM budget and the resident-scene
+ * limit allow, keeps every admitted scene resident and advances all of them
+ * together (one mpm_batch, chunks of chunk_steps batched steps), and retires
+ * each request when its steps are done (final particles copied to host
+ * memory held for mpm_server_result, device memory released).  A request's
+ * result is bit-identical to stepping its scene alone.  Served scenes re-sort
+ * every step, carry no diffuse pool and keep no i32 grids (sort_period,
+ * flags and device of the submitted configuration are set accordingly).
+ */
+typedef struct mpm_server mpm_server;
+
+#define MPM_REQ_QUEUED 0
+#define MPM_REQ_RUNNING 1
+#define MPM_REQ_DONE 2
+#define MPM_REQ_FAILED 3
+#define MPM_REQ_CANCELLED 4
+
+typedef struct mpm_server_config {
+    int32_t device;
+    uint32_t max_active;      /* resident scenes at most (0 = no limit) */
+    uint64_t hbm_budget;      /* device bytes the resident scenes may hold */
+    uint32_t chunk_steps;     /* batched steps per loop iteration (0 = 8) */
+    uint32_t item_blocks;     /* batch launch shapes (mpm_batch_tuning; 0 = default) */
+    uint32_t resolve_blocks;
+    uint32_t aux_blocks;
+    uint32_t reserved[4];
+} mpm_server_config;
+
+typedef struct mpm_server_stats {
+    uint64_t submitted, completed, failed, cancelled;
+    uint32_t queued, active;
+    uint64_t bytes_resident;  /* admission estimate of the resident scenes */
+    uint64_t particle_steps;  /* particle-steps completed on the device */
+    uint64_t batched_steps;
+    double busy_seconds;      /* loop time spent stepping resident scenes */
+    double elapsed_seconds;   /* since mpm_server_create */
+} mpm_server_stats;
+
+typedef struct mpm_request_info {
+    int32_t state;            /* MPM_REQ_* */
+    uint32_t n, steps, steps_done;
+    uint64_t footprint;       /* device bytes reserved at admission */
+    double t_submit, t_admit, t_done;  /* seconds since mpm_server_create */
+} mpm_request_info;
+
+/* Starts the loop thread. */
+int mpm_server_create(const mpm_server_config *cfg, mpm_server **out);
+/* Cancels everything outstanding, joins the loop, frees. */
+int mpm_server_destroy(mpm_server *srv);
+/* Queue a scene: inputs are copied (pos3 n*3, vel3 n*3, C9 n*9, mass n; NULL
+ * = zero / zero / 1.0).  MPM_ERR_NOMEM if the scene alone exceeds the
+ * budget.  Never waits for the device. */
+int mpm_server_submit(mpm_server *srv, const mpm_config *cfg, uint32_t n, const float *pos3,
+                      const float *vel3, const float *C9, const float *mass, uint32_t steps,
+                      uint64_t *id);
+int mpm_server_poll(mpm_server *srv, uint64_t id, mpm_request_info *info);
+/* Block until the request is done, failed or cancelled, or timeout_s passes
+ * (negative = no limit); *state receives its state. */
+int mpm_server_wait(mpm_server *srv, uint64_t id, double timeout_s, int *state);
+/* Copy a finished request's final particles (reset order; layouts of
+ * mpm_read_particles; any output may be NULL) and release the request. */
+int mpm_server_result(mpm_server *srv, uint64_t id, float *pos4, float *vel4, float *C9,
+                      float *density);
+int mpm_server_cancel(mpm_server *srv, uint64_t id);
+int mpm_server_get_stats(mpm_server *srv, mpm_server_stats *out);
+/* GPU time of every batched step so far (HIP events), in milliseconds. */
+int mpm_server_step_times(mpm_server *srv, float *ms, uint32_t cap, uint32_t *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,7 @@ size_t scene_footprint(const mpm_config &c) {
   b += cap * 12 + nb * 16;               // keys, ranks, src_of; counts, offsets, first, count
   b += nit * (16 + 4 * 4000);            // work items and ghost cores
   b += nb * 8 + 65536;                   // flags, lists, scan scratch
+  b += cap * 72;                         // read-back staging
   if (D) b += D * (48 + 4) + cap * 4 + (cap / 32 + D / 32 + 2) * 8 + 4096;
   return b;
 }
@@ -855,13 +856,9 @@ int scene_tile_stats(mpm_scene *s, uint64_t out[4]) {
   HIPCHK(hipMemcpy(&nit, s->nitems, 4, hipMemcpyDeviceToHost));
   std::vector<uint4> it(nit);
   if (nit) HIPCHK(hipMemcpy(it.data(), s->items, (size_t)nit * 16, hipMemcpyDeviceToHost));
-  uint64_t mx = 0, bricks = 0;
-  uint32_t last = 0xffffffffu;
-  for (const uint4 &q : it) {
+  uint64_t mx = 0;
+  for (const uint4 &q : it)
     if (q.z > mx) mx = q.z;
-    if (q.x != last) bricks++;
-    last = q.x;
-  }
   out[0] = nit;
   out[1] = s->bw.nbricks;
   out[2] = s->n;

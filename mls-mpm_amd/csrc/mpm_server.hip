@@ -1,0 +1,439 @@
+// mpm_server.hip -- serving form: a request loop with continuous batching.
+//
+// Requests (a scene configuration, its particles and a step count) are
+// queued by any thread.  One loop thread owns the device: it admits queued
+// requests in arrival order while the HBM budget and the resident-scene
+// limit allow (creating and loading each scene while the device keeps
+// stepping the others), keeps every admitted scene resident in one mpm_batch,
+// advances all of them together in chunks of batched steps (one persistent
+// launch per stage), and retires each request when its steps are done: the
+// final particles are copied to host memory owned by the request and the
+// scene's device memory is released, making room for the next admissions.
+// A request's result is bit-identical to stepping its scene alone: the
+// batched kernels run the single-scene bodies.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mpm_scene.h"
+
+using namespace mpm;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  uint64_t id = 0;
+  mpm_config cfg{};
+  uint32_t n = 0, steps = 0, done = 0;
+  std::vector<float> pos3, vel3, C9, mass;
+  int state = MPM_REQ_QUEUED;
+  bool cancel = false;
+  std::string error;
+  size_t footprint = 0;
+  mpm_scene *scene = nullptr;
+  std::vector<float> pos4, vel4, C9out, density;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+int sfail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int sfail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  return set_error(code, buf);
+}
+
+}  // namespace
+
+struct mpm_server {
+  mpm_server_config cfg{};
+  Clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cv_work;  // loop: new request, cancel or stop
+  std::condition_variable cv_done;  // waiters: a request finished
+  bool stop = false;
+  uint64_t next_id = 1;
+  std::deque<uint64_t> queue;                          // admission order
+  std::map<uint64_t, std::unique_ptr<Request>> reqs;   // every unreleased request
+  std::vector<Request *> active;                       // resident (changed by the loop, under mu)
+  uint64_t bytes_resident = 0;
+  mpm_server_stats stats{};
+  std::vector<float> step_ms;                          // per batched step
+  std::thread loop;
+  mpm_batch *batch = nullptr;
+  int loop_error = 0;
+  std::string loop_msg;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+// Create, load and enroll one request's scene (loop thread).
+int admit(mpm_server *srv, Request *r) {
+  int rc = mpm_create(&r->cfg, &r->scene);
+  if (!rc)
+    rc = mpm_reset(r->scene, r->n, r->pos3.data(), r->vel3.empty() ? nullptr : r->vel3.data(),
+                   r->C9.empty() ? nullptr : r->C9.data(),
+                   r->mass.empty() ? nullptr : r->mass.data());
+  if (!rc) rc = mpm_batch_add(srv->batch, r->scene);
+  if (rc) {
+    std::string msg = mpm_last_error();
+    if (r->scene) mpm_destroy(r->scene);
+    r->scene = nullptr;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    r->state = MPM_REQ_FAILED;
+    r->error = msg;
+    r->t_done = srv->now();
+    srv->bytes_resident -= r->footprint;
+    srv->stats.failed++;
+    srv->cv_done.notify_all();
+    return rc;
+  }
+  // inputs are on the device now
+  std::vector<float>().swap(r->pos3);
+  std::vector<float>().swap(r->vel3);
+  std::vector<float>().swap(r->C9);
+  std::vector<float>().swap(r->mass);
+  std::lock_guard<std::mutex> lk(srv->mu);
+  srv->active.push_back(r);
+  return MPM_OK;
+}
+
+// Read the final state, release the scene (loop thread).
+void retire(mpm_server *srv, Request *r, bool cancelled) {
+  int rc = MPM_OK;
+  std::string msg;
+  if (!cancelled) {
+    r->pos4.resize((size_t)r->n * 4);
+    r->vel4.resize((size_t)r->n * 4);
+    r->C9out.resize((size_t)r->n * 9);
+    r->density.resize(r->n);
+    rc = mpm_read_particles(r->scene, r->pos4.data(), r->vel4.data(), r->C9out.data(),
+                            r->density.data());
+    if (rc) msg = mpm_last_error();
+  }
+  (void)mpm_batch_remove(srv->batch, r->scene);
+  mpm_destroy(r->scene);
+  r->scene = nullptr;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  srv->bytes_resident -= r->footprint;
+  r->t_done = srv->now();
+  if (cancelled) {
+    r->state = MPM_REQ_CANCELLED;
+    srv->stats.cancelled++;
+  } else if (rc) {
+    r->state = MPM_REQ_FAILED;
+    r->error = msg;
+    srv->stats.failed++;
+  } else {
+    r->state = MPM_REQ_DONE;
+    srv->stats.completed++;
+  }
+  srv->cv_done.notify_all();
+}
+
+void run_loop(mpm_server *srv) {
+  if (hipSetDevice(srv->cfg.device) != hipSuccess || mpm_batch_create(srv->cfg.device, &srv->batch)) {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->loop_error = MPM_ERR_HIP;
+    srv->loop_msg = mpm_last_error();
+    srv->stop = true;
+    srv->cv_done.notify_all();
+    return;
+  }
+  mpm_batch_tuning tune{};
+  mpm_batch_get_tuning(srv->batch, &tune);
+  tune.item_blocks = srv->cfg.item_blocks;
+  if (srv->cfg.resolve_blocks) tune.resolve_blocks = srv->cfg.resolve_blocks;
+  if (srv->cfg.aux_blocks) tune.aux_blocks = srv->cfg.aux_blocks;
+  mpm_batch_set_tuning(srv->batch, &tune);
+  mpm_batch_enable_timing(srv->batch, 1);
+  const uint32_t chunk = srv->cfg.chunk_steps ? srv->cfg.chunk_steps : 8;
+  std::vector<float> ms;
+  for (;;) {
+    // Admissions, in arrival order, while the budget and slot limit allow.
+    std::vector<Request *> admitting;
+    {
+      std::unique_lock<std::mutex> lk(srv->mu);
+      if (srv->active.empty() && srv->queue.empty()) {
+        if (srv->stop) break;
+        srv->cv_work.wait_for(lk, std::chrono::milliseconds(20));
+        continue;
+      }
+      while (!srv->queue.empty()) {
+        Request *r = srv->reqs[srv->queue.front()].get();
+        if (r->cancel) {
+          srv->queue.pop_front();
+          r->state = MPM_REQ_CANCELLED;
+          r->t_done = srv->now();
+          srv->stats.cancelled++;
+          srv->cv_done.notify_all();
+          continue;
+        }
+        const size_t resident = srv->active.size() + admitting.size();
+        if (srv->cfg.max_active && resident >= srv->cfg.max_active) break;
+        if (srv->bytes_resident + r->footprint > srv->cfg.hbm_budget) break;
+        srv->queue.pop_front();
+        srv->bytes_resident += r->footprint;
+        r->state = MPM_REQ_RUNNING;
+        r->t_admit = srv->now();
+        admitting.push_back(r);
+      }
+    }
+    for (Request *r : admitting) admit(srv, r);
+    if (srv->active.empty()) continue;
+    // One chunk of batched steps over every resident scene.
+    uint32_t c = chunk;
+    for (Request *r : srv->active) c = std::min(c, r->steps - r->done);
+    const double tb = srv->now();
+    int rc = mpm_batch_step(srv->batch, c);
+    if (!rc) rc = mpm_batch_sync(srv->batch);
+    uint32_t cnt = 0;
+    if (!rc) rc = mpm_batch_step_times(srv->batch, nullptr, 0, &cnt);
+    if (!rc && cnt) {
+      ms.resize(cnt);
+      rc = mpm_batch_step_times(srv->batch, ms.data(), cnt, &cnt);
+      mpm_batch_enable_timing(srv->batch, 1);  // restart the record
+    }
+    if (rc) {
+      std::string msg = mpm_last_error();
+      std::vector<Request *> act;
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        act.swap(srv->active);
+      }
+      for (Request *r : act) {
+        (void)mpm_batch_remove(srv->batch, r->scene);
+        mpm_destroy(r->scene);
+        r->scene = nullptr;
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->bytes_resident -= r->footprint;
+        r->state = MPM_REQ_FAILED;
+        r->error = msg;
+        r->t_done = srv->now();
+        srv->stats.failed++;
+      }
+      std::lock_guard<std::mutex> lk(srv->mu);
+      srv->cv_done.notify_all();
+      continue;
+    }
+    std::vector<Request *> finished, cancelled;
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      uint64_t ps = 0;
+      for (Request *r : srv->active) {
+        r->done += c;
+        ps += (uint64_t)r->n * c;
+        if (r->cancel) cancelled.push_back(r);
+        else if (r->done >= r->steps) finished.push_back(r);
+      }
+      srv->stats.particle_steps += ps;
+      srv->stats.batched_steps += c;
+      srv->stats.busy_seconds += srv->now() - tb;
+      srv->step_ms.insert(srv->step_ms.end(), ms.begin(), ms.begin() + std::min<size_t>(cnt, ms.size()));
+    }
+    for (Request *r : finished) retire(srv, r, false);
+    for (Request *r : cancelled) retire(srv, r, true);
+    if (!finished.empty() || !cancelled.empty()) {
+      auto keep = [&](Request *r) {
+        return std::find(finished.begin(), finished.end(), r) == finished.end() &&
+               std::find(cancelled.begin(), cancelled.end(), r) == cancelled.end();
+      };
+      std::lock_guard<std::mutex> lk(srv->mu);
+      std::vector<Request *> rest;
+      for (Request *r : srv->active)
+        if (keep(r)) rest.push_back(r);
+      srv->active.swap(rest);
+    }
+  }
+  mpm_batch_destroy(srv->batch);
+  srv->batch = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_server_create(const mpm_server_config *cfg, mpm_server **out) {
+  if (!cfg || !out) return sfail(MPM_ERR_INVALID, "mpm_server_create: NULL argument");
+  *out = nullptr;
+  if (!cfg->hbm_budget) return sfail(MPM_ERR_INVALID, "mpm_server_create: hbm_budget is 0");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return sfail(MPM_ERR_HIP, "no HIP device available (%s)",
+                 e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return sfail(MPM_ERR_INVALID, "device %d out of range (%d devices)", cfg->device, ndev);
+  mpm_server *srv = new mpm_server();
+  srv->cfg = *cfg;
+  srv->t0 = Clock::now();
+  srv->loop = std::thread(run_loop, srv);
+  *out = srv;
+  return MPM_OK;
+}
+
+int mpm_server_destroy(mpm_server *srv) {
+  if (!srv) return MPM_OK;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stop = true;
+    for (auto &kv : srv->reqs) kv.second->cancel = true;
+    srv->cv_work.notify_all();
+  }
+  if (srv->loop.joinable()) srv->loop.join();
+  delete srv;
+  return MPM_OK;
+}
+
+int mpm_server_submit(mpm_server *srv, const mpm_config *cfg, uint32_t n, const float *pos3,
+                      const float *vel3, const float *C9, const float *mass, uint32_t steps,
+                      uint64_t *id) {
+  if (!srv || !cfg || !id || (n && !pos3))
+    return sfail(MPM_ERR_INVALID, "mpm_server_submit: NULL argument");
+  mpm_config c = *cfg;
+  c.device = srv->cfg.device;
+  c.sort_period = 1;
+  c.flags &= ~(MPM_FLAG_NO_LDS | MPM_FLAG_KEEP_GRID);
+  if (c.diffuse_capacity)
+    return sfail(MPM_ERR_UNSUPPORTED, "served scenes carry no diffuse pool");
+  if (n > c.capacity) return sfail(MPM_ERR_CAPACITY, "%u particles > capacity %u", n, c.capacity);
+  int rc = validate_config(c);
+  if (rc) return rc;
+  auto r = std::make_unique<Request>();
+  r->cfg = c;
+  r->n = n;
+  r->steps = steps;
+  r->footprint = scene_footprint(c);
+  if (r->footprint > srv->cfg.hbm_budget)
+    return sfail(MPM_ERR_NOMEM, "scene needs %zu bytes, budget %llu", r->footprint,
+                 (unsigned long long)srv->cfg.hbm_budget);
+  r->pos3.assign(pos3, pos3 + (size_t)n * 3);
+  if (vel3) r->vel3.assign(vel3, vel3 + (size_t)n * 3);
+  if (C9) r->C9.assign(C9, C9 + (size_t)n * 9);
+  if (mass) r->mass.assign(mass, mass + n);
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (srv->stop || srv->loop_error)
+    return sfail(MPM_ERR_STATE, "server is stopped%s%s", srv->loop_msg.empty() ? "" : ": ",
+                 srv->loop_msg.c_str());
+  r->id = srv->next_id++;
+  r->t_submit = srv->now();
+  *id = r->id;
+  srv->queue.push_back(r->id);
+  srv->reqs[r->id] = std::move(r);
+  srv->stats.submitted++;
+  srv->cv_work.notify_all();
+  return MPM_OK;
+}
+
+int mpm_server_poll(mpm_server *srv, uint64_t id, mpm_request_info *info) {
+  if (!srv || !info) return sfail(MPM_ERR_INVALID, "mpm_server_poll: NULL argument");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->reqs.find(id);
+  if (it == srv->reqs.end()) return sfail(MPM_ERR_NOT_FOUND, "request %llu unknown", (unsigned long long)id);
+  const Request &r = *it->second;
+  info->state = r.state;
+  info->steps = r.steps;
+  info->steps_done = r.done;
+  info->n = r.n;
+  info->t_submit = r.t_submit;
+  info->t_admit = r.t_admit;
+  info->t_done = r.t_done;
+  info->footprint = r.footprint;
+  return MPM_OK;
+}
+
+int mpm_server_wait(mpm_server *srv, uint64_t id, double timeout_s, int *state) {
+  if (!srv) return sfail(MPM_ERR_INVALID, "mpm_server_wait: server is NULL");
+  std::unique_lock<std::mutex> lk(srv->mu);
+  auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(
+                                     std::chrono::duration<double>(timeout_s < 0 ? 1e9 : timeout_s));
+  for (;;) {
+    auto it = srv->reqs.find(id);
+    if (it == srv->reqs.end())
+      return sfail(MPM_ERR_NOT_FOUND, "request %llu unknown", (unsigned long long)id);
+    const int st = it->second->state;
+    if (state) *state = st;
+    if (st == MPM_REQ_DONE || st == MPM_REQ_FAILED || st == MPM_REQ_CANCELLED) return MPM_OK;
+    if (srv->loop_error) return sfail(srv->loop_error, "server loop failed: %s", srv->loop_msg.c_str());
+    if (srv->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
+      if (state) *state = srv->reqs.count(id) ? srv->reqs[id]->state : MPM_REQ_FAILED;
+      return MPM_OK;
+    }
+  }
+}
+
+int mpm_server_result(mpm_server *srv, uint64_t id, float *pos4, float *vel4, float *C9,
+                      float *density) {
+  if (!srv) return sfail(MPM_ERR_INVALID, "mpm_server_result: server is NULL");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->reqs.find(id);
+  if (it == srv->reqs.end())
+    return sfail(MPM_ERR_NOT_FOUND, "request %llu unknown", (unsigned long long)id);
+  Request &r = *it->second;
+  if (r.state == MPM_REQ_FAILED) {
+    std::string msg = r.error;
+    srv->reqs.erase(it);
+    return sfail(MPM_ERR_STATE, "request failed: %s", msg.c_str());
+  }
+  if (r.state == MPM_REQ_CANCELLED) {
+    srv->reqs.erase(it);
+    return sfail(MPM_ERR_STATE, "request was cancelled");
+  }
+  if (r.state != MPM_REQ_DONE) return sfail(MPM_ERR_STATE, "request not finished");
+  if (pos4) memcpy(pos4, r.pos4.data(), r.pos4.size() * 4);
+  if (vel4) memcpy(vel4, r.vel4.data(), r.vel4.size() * 4);
+  if (C9) memcpy(C9, r.C9out.data(), r.C9out.size() * 4);
+  if (density) memcpy(density, r.density.data(), r.density.size() * 4);
+  srv->reqs.erase(it);
+  return MPM_OK;
+}
+
+int mpm_server_cancel(mpm_server *srv, uint64_t id) {
+  if (!srv) return sfail(MPM_ERR_INVALID, "mpm_server_cancel: server is NULL");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->reqs.find(id);
+  if (it == srv->reqs.end())
+    return sfail(MPM_ERR_NOT_FOUND, "request %llu unknown", (unsigned long long)id);
+  it->second->cancel = true;
+  srv->cv_work.notify_all();
+  return MPM_OK;
+}
+
+int mpm_server_get_stats(mpm_server *srv, mpm_server_stats *out) {
+  if (!srv || !out) return sfail(MPM_ERR_INVALID, "mpm_server_get_stats: NULL argument");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  *out = srv->stats;
+  out->queued = (uint32_t)srv->queue.size();
+  out->active = (uint32_t)srv->active.size();
+  out->bytes_resident = srv->bytes_resident;
+  out->elapsed_seconds = srv->now();
+  return MPM_OK;
+}
+
+int mpm_server_step_times(mpm_server *srv, float *ms, uint32_t cap, uint32_t *count) {
+  if (!srv) return sfail(MPM_ERR_INVALID, "mpm_server_step_times: server is NULL");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  const uint32_t n = (uint32_t)srv->step_ms.size();
+  if (count) *count = n;
+  if (ms)
+    for (uint32_t i = 0; i < n && i < cap; i++) ms[i] = srv->step_ms[i];
+  return MPM_OK;
+}
+
+}  // extern "C"

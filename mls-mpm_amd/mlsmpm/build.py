@@ -58,7 +58,7 @@ def build_library(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(len(cmds), 6)) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

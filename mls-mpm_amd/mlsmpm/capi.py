@@ -76,6 +76,33 @@ class BatchTuning(C.Structure):
                 ("reserved", C.c_uint32 * 3)]
 
 
+MPM_REQ_QUEUED, MPM_REQ_RUNNING, MPM_REQ_DONE, MPM_REQ_FAILED, MPM_REQ_CANCELLED = range(5)
+
+
+class ServerConfig(C.Structure):
+    """mpm_server_config."""
+    _fields_ = [("device", C.c_int32), ("max_active", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("chunk_steps", C.c_uint32), ("item_blocks", C.c_uint32),
+                ("resolve_blocks", C.c_uint32), ("aux_blocks", C.c_uint32),
+                ("reserved", C.c_uint32 * 4)]
+
+
+class ServerStats(C.Structure):
+    """mpm_server_stats."""
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("cancelled", C.c_uint64), ("queued", C.c_uint32), ("active", C.c_uint32),
+                ("bytes_resident", C.c_uint64), ("particle_steps", C.c_uint64),
+                ("batched_steps", C.c_uint64), ("busy_seconds", C.c_double),
+                ("elapsed_seconds", C.c_double)]
+
+
+class RequestInfo(C.Structure):
+    """mpm_request_info."""
+    _fields_ = [("state", C.c_int32), ("n", C.c_uint32), ("steps", C.c_uint32),
+                ("steps_done", C.c_uint32), ("footprint", C.c_uint64), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
 # Every entry point declared in include/mlsmpm.h, with its argument types.
 _VP = C.c_void_p
 _SIGNATURES = {
@@ -114,6 +141,15 @@ _SIGNATURES = {
     "mpm_batch_sync": ([_VP], C.c_int),
     "mpm_batch_enable_timing": ([_VP, C.c_int], C.c_int),
     "mpm_batch_step_times": ([_VP, _VP, C.c_uint32, _VP], C.c_int),
+    "mpm_server_create": ([_VP, _VP], C.c_int),
+    "mpm_server_destroy": ([_VP], C.c_int),
+    "mpm_server_submit": ([_VP, _VP, C.c_uint32, _VP, _VP, _VP, _VP, C.c_uint32, _VP], C.c_int),
+    "mpm_server_poll": ([_VP, C.c_uint64, _VP], C.c_int),
+    "mpm_server_wait": ([_VP, C.c_uint64, C.c_double, _VP], C.c_int),
+    "mpm_server_result": ([_VP, C.c_uint64, _VP, _VP, _VP, _VP], C.c_int),
+    "mpm_server_cancel": ([_VP, C.c_uint64], C.c_int),
+    "mpm_server_get_stats": ([_VP, _VP], C.c_int),
+    "mpm_server_step_times": ([_VP, _VP, C.c_uint32, _VP], C.c_int),
 }
 
 _lib = None

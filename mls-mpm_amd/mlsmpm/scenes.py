@@ -105,3 +105,25 @@ def scaled_dam_break(grid, n_target: int, rng: np.random.Generator, kind: str = 
         return np.clip(pos, 1.0, None), vel
     pos = lattice_block((2.0, 2.0, 2.0), (2.0 + w, 2.0 + h, 2.0 + zlen), 0.1, rng)
     return pos, None
+
+
+def scene_mix(seed: int, count: int, edge=(32, 128), n_range=(10_000, 2_000_000),
+              kinds=("dam",)):
+    """Seeded mix of independent scenes: every grid edge drawn log-uniform
+    (integer) in `edge` per axis, particle target log-uniform in `n_range`,
+    scene kind drawn from `kinds` ('dam' column or 'two' approaching blocks),
+    sized by scaled_dam_break (so capped at 8 particles per cell over half
+    the interior; the realized count is len(pos)).  Returns a list of
+    (grid, pos, vel, kind)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    lo_e, hi_e = math.log(edge[0]), math.log(edge[1] + 1)
+    lo_n, hi_n = math.log(n_range[0]), math.log(n_range[1])
+    for _ in range(count):
+        g = tuple(int(min(edge[1], max(edge[0], math.floor(math.exp(rng.uniform(lo_e, hi_e))))))
+                  for _ in range(3))
+        n = int(math.exp(rng.uniform(lo_n, hi_n)))
+        kind = kinds[int(rng.integers(len(kinds)))]
+        pos, vel = scaled_dam_break(g, n, rng, kind)
+        out.append((g, pos, vel, kind))
+    return out

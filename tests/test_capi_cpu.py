@@ -97,3 +97,26 @@ def test_null_handles_are_rejected():
     assert L.mpm_step(None, 1) == capi.MPM_ERR_INVALID
     assert b"NULL" in L.mpm_last_error()
     assert L.mpm_destroy(None) == capi.MPM_OK
+
+
+def test_batch_and_server_fail_loudly_without_device():
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(mlsmpm.MpmError) as e:
+        mlsmpm.Batch()
+    assert e.value.code == capi.MPM_ERR_HIP
+    with pytest.raises(mlsmpm.MpmError) as e:
+        mlsmpm.Server(hbm_budget=1 << 30)
+    assert e.value.code == capi.MPM_ERR_HIP
+
+
+def test_abi_struct_sizes():
+    assert C.sizeof(capi.BatchTuning) == 8 * 4
+    assert C.sizeof(capi.ServerConfig) == 48
+    assert C.sizeof(capi.ServerStats) == 4 * 8 + 2 * 4 + 3 * 8 + 2 * 8
+    assert C.sizeof(capi.RequestInfo) == 4 * 4 + 8 + 3 * 8
