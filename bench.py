@@ -183,7 +183,7 @@ def serve_bench(args) -> dict:
     wall time from first arrival to last completion), p50/p99 batched-step
     latency and p50/p99 request latency; `vs_plain` divides the busy-time rate
     by the same requests' plain single-scene rate (each scene's per-step time
-    measured alone, times its step count, summed)."""
+    measured alone on a seeded sample of 48 requests, times its step count)."""
     import mlsmpm
     from mlsmpm import scenes
 
@@ -213,9 +213,12 @@ def serve_bench(args) -> dict:
     st = srv.stats()
     step_ms = srv.step_times()
     srv.close()
-    # plain form: every scene alone, per-step time over 32 graph-replayed steps
-    plain_s = 0.0
-    for (grid, pos, vel, _), k in zip(mix, steps):
+    # plain form: a seeded sample of the requests, each scene alone, per-step
+    # time over 32 graph-replayed steps
+    sample = np.random.default_rng(args.seed + 13).choice(nreq, min(nreq, 48), replace=False)
+    plain_s, plain_work = 0.0, 0
+    for i in sample:
+        grid, pos, vel, _ = mix[i]
         with mlsmpm.Scene(mlsmpm.default_config(*grid, len(pos), sort_period=1)) as s:
             s.reset(pos, vel)
             s.step(17)
@@ -223,7 +226,9 @@ def serve_bench(args) -> dict:
             ta = time.perf_counter()
             s.step(32)
             s.sync()
-            plain_s += (time.perf_counter() - ta) / 32 * int(k)
+            plain_s += (time.perf_counter() - ta) / 32 * int(steps[i])
+            plain_work += len(pos) * int(steps[i])
+    plain_rate = plain_work / plain_s
     busy_rate = st["particle_steps"] / st["busy_seconds"]
     return {
         "requests": nreq,
@@ -234,8 +239,10 @@ def serve_bench(args) -> dict:
         "value": busy_rate,
         "unit": "particle-steps/s",
         "wall_rate": st["particle_steps"] / wall,
-        "plain_rate": work / plain_s,
-        "vs_plain": busy_rate / (work / plain_s),
+        "plain_rate": plain_rate,
+        "plain_sample": int(len(sample)),
+        "vs_plain": busy_rate / plain_rate,
+        "mean_resident": st["resident_steps"] / max(1, st["batched_steps"]),
         "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                             "p99": float(np.percentile(step_ms, 99))},
         "request_latency_s": {"p50": float(np.percentile(lat, 50)),
@@ -264,9 +271,9 @@ def main() -> int:
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=48)
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=12.0)
-    ap.add_argument("--serve-seconds", type=float, default=15.0)
-    ap.add_argument("--serve-chunk", type=int, default=8)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-chunk", type=int, default=16)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES each (the plain copy that "

@@ -264,6 +264,7 @@ typedef struct mpm_server_stats {
     uint64_t bytes_resident;  /* admission estimate of the resident scenes */
     uint64_t particle_steps;  /* particle-steps completed on the device */
     uint64_t batched_steps;
+    uint64_t resident_steps;  /* sum over batched steps of the resident scenes */
     double busy_seconds;      /* loop time spent stepping resident scenes */
     double elapsed_seconds;   /* since mpm_server_create */
 } mpm_server_stats;

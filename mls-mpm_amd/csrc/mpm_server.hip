@@ -83,14 +83,22 @@ struct mpm_server {
 
 namespace {
 
-// Create, load and enroll one request's scene (loop thread).
-int admit(mpm_server *srv, Request *r) {
+// Create and load one request's scene and take its first step (the sort of
+// the caller's order) on the scene's own stream, while the batch runs
+// (loop thread).
+int prepare(mpm_server *srv, Request *r) {
   int rc = mpm_create(&r->cfg, &r->scene);
   if (!rc)
     rc = mpm_reset(r->scene, r->n, r->pos3.data(), r->vel3.empty() ? nullptr : r->vel3.data(),
                    r->C9.empty() ? nullptr : r->C9.data(),
                    r->mass.empty() ? nullptr : r->mass.data());
-  if (!rc) rc = mpm_batch_add(srv->batch, r->scene);
+  if (!rc && r->steps) {
+    rc = mpm_step(r->scene, 1);
+    if (!rc) {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      r->done = 1;
+    }
+  }
   if (rc) {
     std::string msg = mpm_last_error();
     if (r->scene) mpm_destroy(r->scene);
@@ -109,6 +117,25 @@ int admit(mpm_server *srv, Request *r) {
   std::vector<float>().swap(r->vel3);
   std::vector<float>().swap(r->C9);
   std::vector<float>().swap(r->mass);
+  return MPM_OK;
+}
+
+// Join a prepared scene to the batch (after the chunk it overlapped).
+int enroll(mpm_server *srv, Request *r) {
+  int rc = mpm_batch_add(srv->batch, r->scene);
+  if (rc) {
+    std::string msg = mpm_last_error();
+    mpm_destroy(r->scene);
+    r->scene = nullptr;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    r->state = MPM_REQ_FAILED;
+    r->error = msg;
+    r->t_done = srv->now();
+    srv->bytes_resident -= r->footprint;
+    srv->stats.failed++;
+    srv->cv_done.notify_all();
+    return rc;
+  }
   std::lock_guard<std::mutex> lk(srv->mu);
   srv->active.push_back(r);
   return MPM_OK;
@@ -195,13 +222,26 @@ void run_loop(mpm_server *srv) {
         admitting.push_back(r);
       }
     }
-    for (Request *r : admitting) admit(srv, r);
-    if (srv->active.empty()) continue;
-    // One chunk of batched steps over every resident scene.
+    // One chunk of batched steps over every resident scene, enqueued first;
+    // the admitted scenes are created, loaded and sorted while it runs.
+    const bool stepping = !srv->active.empty();
     uint32_t c = chunk;
     for (Request *r : srv->active) c = std::min(c, r->steps - r->done);
     const double tb = srv->now();
-    int rc = mpm_batch_step(srv->batch, c);
+    int rc = stepping ? mpm_batch_step(srv->batch, c) : MPM_OK;
+    std::vector<Request *> prepared;
+    for (Request *r : admitting)
+      if (prepare(srv, r) == MPM_OK) prepared.push_back(r);
+    auto join_prepared = [&]() {
+      for (Request *r : prepared) {
+        if (r->done >= r->steps) retire(srv, r, false);
+        else enroll(srv, r);
+      }
+    };
+    if (!stepping) {
+      join_prepared();
+      continue;
+    }
     if (!rc) rc = mpm_batch_sync(srv->batch);
     uint32_t cnt = 0;
     if (!rc) rc = mpm_batch_step_times(srv->batch, nullptr, 0, &cnt);
@@ -228,8 +268,11 @@ void run_loop(mpm_server *srv) {
         r->t_done = srv->now();
         srv->stats.failed++;
       }
-      std::lock_guard<std::mutex> lk(srv->mu);
-      srv->cv_done.notify_all();
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->cv_done.notify_all();
+      }
+      join_prepared();
       continue;
     }
     std::vector<Request *> finished, cancelled;
@@ -244,6 +287,7 @@ void run_loop(mpm_server *srv) {
       }
       srv->stats.particle_steps += ps;
       srv->stats.batched_steps += c;
+      srv->stats.resident_steps += (uint64_t)srv->active.size() * c;
       srv->stats.busy_seconds += srv->now() - tb;
       srv->step_ms.insert(srv->step_ms.end(), ms.begin(), ms.begin() + std::min<size_t>(cnt, ms.size()));
     }
@@ -260,6 +304,7 @@ void run_loop(mpm_server *srv) {
         if (keep(r)) rest.push_back(r);
       srv->active.swap(rest);
     }
+    join_prepared();
   }
   mpm_batch_destroy(srv->batch);
   srv->batch = nullptr;

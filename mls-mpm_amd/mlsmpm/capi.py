@@ -92,7 +92,8 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("cancelled", C.c_uint64), ("queued", C.c_uint32), ("active", C.c_uint32),
                 ("bytes_resident", C.c_uint64), ("particle_steps", C.c_uint64),
-                ("batched_steps", C.c_uint64), ("busy_seconds", C.c_double),
+                ("batched_steps", C.c_uint64), ("resident_steps", C.c_uint64),
+                ("busy_seconds", C.c_double),
                 ("elapsed_seconds", C.c_double)]
 
 
