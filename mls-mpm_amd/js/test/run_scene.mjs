@@ -1,0 +1,43 @@
+// Runs a small scene through FluidHIP + DiffuseHIP on the GPU and dumps its
+// inputs, the wall uniform of every frame and the state after every frame
+// (raw little-endian float32 / bytes) into the directory given as argv[2],
+// for comparison with the CPU restatement.
+import fs from 'fs';
+import path from 'path';
+import { FluidHIP } from '../fluid-hip.mjs';
+import { DiffuseHIP } from '../diffuse-hip.mjs';
+
+const dir = process.argv[2];
+const frames = parseInt(process.argv[3] || '6', 10);
+let s = 12345;
+const rnd = () => {  // seeded jitter source (LCG, 2^32)
+    s = (Math.imul(s, 1664525) + 1013904223) >>> 0;
+    return s / 4294967296;
+};
+const f = new FluidHIP(2, 2, 3, 0.025, 30000);
+f.random = rnd;
+const n = f.fillBlock(0.1, 0.1, 0.1, 0.6, 0.8, 0.5);
+const put = (name, a) => fs.writeFileSync(path.join(dir, name), Buffer.from(a.buffer, a.byteOffset, a.byteLength));
+put('pos3.f32', f.particle_pos.subarray(0, 3 * n));
+f.initGPU(0);
+const d = new DiffuseHIP(8192);
+d.initGPU(0, f);
+f.diffuse = d;
+f.setWall(0, [13.0, 2.0, 4.0], [15.0, 11.0, 20.0]);
+const walls = [];
+for (let k = 0; k < frames; k++) {
+    if (k >= 1) f.setWall(0, [13.0 - 0.3 * k, 2.0, 4.0], [15.0 - 0.3 * k, 11.0, 20.0]);
+    if (k >= 2) f.rotateWall(0, 0.05, 0.0);
+    if (!f.simFrame()) throw f.lastError;
+    walls.push(Float32Array.from(f._wallF32));
+    put(`pos4_${k}.f32`, f.readPositions());
+    put(`vel4_${k}.f32`, f.readVelocities());
+}
+const wall = new Float32Array(walls.length * walls[0].length);
+walls.forEach((w, i) => wall.set(w, i * w.length));
+put('walls.f32', wall);
+const pool = new Uint8Array(d.readParticles());
+put('pool.bin', pool);
+f.sync();
+console.log(JSON.stringify({ n, frames, alive: d.aliveCount, grid: [f.grid_X_num, f.grid_Y_num, f.grid_Z_num] }));
+f.destroy();

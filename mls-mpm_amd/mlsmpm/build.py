@@ -75,6 +75,7 @@ def build_addon(verbose: bool = False) -> str | None:
         return None
     out = os.path.join(PKG_ROOT, "js", "addon", "mlsmpm.node")
     _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", "-Wextra", "-Wno-unused-parameter",
+          "-DNODE_GYP_MODULE_NAME=mlsmpm",
           f"-I{node_inc}", f"-I{os.path.join(REPO_ROOT, 'include')}", "-o", out, src,
           f"-L{LIBDIR}", "-lmlsmpm", f"-Wl,-rpath,$ORIGIN/../../lib"])
     if verbose:
