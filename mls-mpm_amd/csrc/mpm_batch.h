@@ -54,7 +54,9 @@ struct BatchTables {
 
 // One batched step (sort period 1: every step re-sorts inside G2P) of every
 // scene of the table, all with drift margin `margin`.
-void launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &grid,
-                       hipStream_t st);
+// The re-sort preparation runs on `side`, forked after the mass pass with
+// event `fork` and joined before G2P with `join`.
+int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &grid,
+                      hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 
 }  // namespace mpm

@@ -35,11 +35,6 @@ __device__ __forceinline__ uint32_t find_scene(const uint32_t *__restrict__ pre,
   return lo;
 }
 
-__global__ __launch_bounds__(256) void k_batch_zero(const SceneDev *__restrict__ sd, uint32_t ns) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ns) *sd[i].bw.nlist = 0u;
-}
-
 // UNIT: every scene of the table has unit particle mass (the general body
 // gives the same bits for unit mass, multiplying by 1.0f exactly).
 template <int M, bool UNIT>
@@ -112,61 +107,14 @@ __global__ __launch_bounds__(512) void k_batch_resolve_update(const SceneDev *__
   }
 }
 
-// Per scene (one block each, persistent): offsets = exclusive scan of the
-// brick counts, counts cleared for the next step, work-item count reset.
-__global__ __launch_bounds__(256) void k_batch_scan(const SceneDev *__restrict__ sd, uint32_t ns) {
-  __shared__ uint32_t wsum[4];
-  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+// Per scene (one block each, persistent): brick offsets and work items of
+// the new order, counts cleared, resolve list emptied.
+__global__ __launch_bounds__(256) void k_batch_scan_items(const SceneDev *__restrict__ sd,
+                                                          uint32_t ns) {
   for (uint32_t si = blockIdx.x; si < ns; si += gridDim.x) {
     const SceneDev &d = sd[si];
-    const uint32_t n = d.bw.nbricks;
-    uint32_t *counts = d.ko.counts, *offsets = d.offsets;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += 1024u) {
-      const uint32_t i0 = base + t * 4u;
-      uint32_t a[4], sum = 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        a[k] = (i0 + k < n) ? counts[i0 + k] : 0u;
-        sum += a[k];
-      }
-      uint32_t incl = sum;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t v = __shfl_up(incl, off, 64);
-        if (lane >= (uint32_t)off) incl += v;
-      }
-      if (lane == 63u) wsum[wave] = incl;
-      __syncthreads();
-      uint32_t woff = 0, tot = 0;
-#pragma unroll
-      for (uint32_t w = 0; w < 4; w++) {
-        if (w < wave) woff += wsum[w];
-        tot += wsum[w];
-      }
-      uint32_t e = carry + woff + incl - sum;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        if (i0 + k < n) {
-          offsets[i0 + k] = e;
-          counts[i0 + k] = 0u;
-        }
-        e += a[k];
-      }
-      carry += tot;
-      __syncthreads();
-    }
-    if (t == 0) *d.bw.nitems = 0u;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_batch_make_items(const SceneDev *__restrict__ sd,
-                                                          const uint32_t *__restrict__ pre,
-                                                          uint32_t ns, uint32_t total) {
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const uint32_t si = find_scene(pre, ns, u);
-    const SceneDev &d = sd[si];
-    make_items_one(d.P.n, d.bw, d.ch, d.offsets, (u - pre[si]) * 256u + threadIdx.x);
+    scan_items_block<16>(d.bw, (uint32_t)__builtin_ctz(d.ch), d.ko.counts, d.offsets);
+    __syncthreads();
   }
 }
 
@@ -185,11 +133,10 @@ static inline uint32_t grid_of(uint32_t total, uint32_t cap) {
   return (cap && cap < total) ? cap : total;
 }
 
-void launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &grid,
-                       hipStream_t st) {
+int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &grid,
+                      hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
   const uint32_t ns = t.ns;
-  if (!ns) return;
-  k_batch_zero<<<(ns + 255) / 256, 256, 0, st>>>(t.sd, ns);
+  if (!ns) return MPM_OK;
   const uint32_t gi = grid_of(t.tot_items, grid.items);
   const uint32_t gr = grid_of(t.tot_bricks, grid.resolve);
   const uint32_t gb = grid_of(t.tot_bblocks, grid.aux);
@@ -202,15 +149,24 @@ void launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &g
     else K<2, false><<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);                         \
   } while (0)
   MPM_BATCH_ITEMS(k_batch_mass);
+  // the re-sort preparation needs only the mass pass: it overlaps the
+  // resolves and P2G momentum on the side stream
+  hipError_t e = hipEventRecord(fork, st);
+  if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+  if (e != hipSuccess) return hip_fail(e, "launch_batch_step fork");
+  k_batch_scan_items<<<std::min<uint32_t>(ns, 2048), 256, 0, side>>>(t.sd, ns);
+  k_batch_build_src<<<gp, 256, 0, side>>>(t.sd, t.pre_pblocks, ns, t.tot_pblocks);
+  e = hipEventRecord(join, side);
+  if (e != hipSuccess) return hip_fail(e, "launch_batch_step join");
   k_batch_brick_list<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks);
   k_batch_resolve_mass<<<gr, 512, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
   MPM_BATCH_ITEMS(k_batch_mom);
 #undef MPM_BATCH_ITEMS
   k_batch_resolve_update<<<gr, 512, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
-  k_batch_scan<<<std::min<uint32_t>(ns, 2048), 256, 0, st>>>(t.sd, ns);
-  k_batch_make_items<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks);
-  k_batch_build_src<<<gp, 256, 0, st>>>(t.sd, t.pre_pblocks, ns, t.tot_pblocks);
+  e = hipStreamWaitEvent(st, join, 0);
+  if (e != hipSuccess) return hip_fail(e, "launch_batch_step wait");
   k_batch_g2p<<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);
+  return MPM_OK;
 }
 
 }  // namespace mpm
@@ -222,6 +178,8 @@ using namespace mpm;
 struct mpm_batch {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;              // re-sort preparation, forked per step
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   mpm_batch_tuning tune{};
   std::vector<mpm_scene *> mem;     // members
   std::vector<uint64_t> seen;       // member versions the tables were built from
@@ -365,10 +323,16 @@ uint32_t graph_len_of(const mpm_batch *b) {
 int capture(mpm_batch *b, int ph) {
   const uint32_t L = graph_len_of(b);
   BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  for (uint32_t i = 0; i < L; i++)
-    launch_batch_step(b->tab[(ph + (int)i) & 1], b->margin, grid_of(b), b->stream);
+  int rc = MPM_OK;
+  for (uint32_t i = 0; i < L && !rc; i++)
+    rc = launch_batch_step(b->tab[(ph + (int)i) & 1], b->margin, grid_of(b), b->stream, b->side,
+                           b->ev_fork, b->ev_join);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
   if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
   e = hipGraphInstantiate(&b->graph[ph], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -419,6 +383,9 @@ int mpm_batch_create(int device, mpm_batch **out) {
   b->tune.aux_blocks = 2048;
   b->tune.graph_steps = 16;
   hipError_t ce = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (ce == hipSuccess) ce = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+  if (ce == hipSuccess) ce = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+  if (ce == hipSuccess) ce = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
   for (uint32_t i = 0; ce == hipSuccess && i < 2 * mpm_batch::kEv; i++)
     ce = hipEventCreate(&b->ev[i]);
   if (ce != hipSuccess) {
@@ -438,6 +405,10 @@ int mpm_batch_destroy(mpm_batch *b) {
   if (b->dev) (void)hipFree(b->dev);
   for (hipEvent_t &e : b->ev)
     if (e) (void)hipEventDestroy(e);
+  if (b->side) (void)hipStreamSynchronize(b->side);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return MPM_OK;
@@ -543,7 +514,9 @@ int mpm_batch_step(mpm_batch *b, uint32_t nsteps) {
       }
       BHIP(hipEventRecord(b->ev[2 * b->ev_used], b->stream));
     }
-    launch_batch_step(b->tab[b->phase], b->margin, grid_of(b), b->stream);
+    int rc = launch_batch_step(b->tab[b->phase], b->margin, grid_of(b), b->stream, b->side,
+                               b->ev_fork, b->ev_join);
+    if (rc) return rc;
     BHIP(hipGetLastError());
     if (b->timing) {
       BHIP(hipEventRecord(b->ev[2 * b->ev_used + 1], b->stream));

@@ -23,8 +23,16 @@ struct mpm_scene {
   mpm::GState g{};
   mpm::DiffuseBufs d{};
   mpm::BrickWork bw{};
-  uint4 *items = nullptr;
-  uint32_t *nitems = nullptr;
+  // the two work-item sets (current = index cur, alternating with the
+  // particle buffers)
+  uint4 *items_buf[2] = {nullptr, nullptr};
+  uint32_t *nitems_buf[2] = {nullptr, nullptr};
+  uint32_t *first_buf[2] = {nullptr, nullptr}, *count_buf[2] = {nullptr, nullptr};
+  uint8_t *occ_buf[2] = {nullptr, nullptr};
+  // the re-sort preparation runs on `side`, forked after P2G mass and joined
+  // before G2P
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   uint32_t *brick_of_rank = nullptr;
   mpm::SortBufs sort{};
   mpm::Walls walls{};
@@ -98,5 +106,6 @@ int scene_step_times(mpm_scene *s, float *ms, uint32_t cap, uint32_t *count);
 Params scene_params(const mpm_scene *s);
 size_t scene_footprint(const mpm_config &c);
 void scene_dev(const mpm_scene *s, int phase, SceneDev *d);
+BrickWork bw_of(const mpm_scene *s, int c);
 int batch_detach(mpm_scene *s);
 }  // namespace mpm

@@ -153,6 +153,22 @@ static PState pstate(const mpm_scene *s, int which) {
   return S;
 }
 
+// Brick work with the current work-item set c and the next set c ^ 1.
+BrickWork bw_of(const mpm_scene *s, int c) {
+  BrickWork b = s->bw;
+  b.items = s->items_buf[c];
+  b.items_w = s->items_buf[c ^ 1];
+  b.nitems = s->nitems_buf[c];
+  b.nitems_w = s->nitems_buf[c ^ 1];
+  b.first = s->first_buf[c];
+  b.first_w = s->first_buf[c ^ 1];
+  b.count = s->count_buf[c];
+  b.count_w = s->count_buf[c ^ 1];
+  b.occ = s->occ_buf[c];
+  b.occ_w = s->occ_buf[c ^ 1];
+  return b;
+}
+
 // Descriptor of the scene for the batched kernels; phase 1 = the buffer
 // parity after one more re-sorting step.
 void scene_dev(const mpm_scene *s, int phase, SceneDev *d) {
@@ -162,8 +178,8 @@ void scene_dev(const mpm_scene *s, int phase, SceneDev *d) {
   d->S = pstate(s, c);
   d->D = pstate(s, c ^ 1);
   d->g = s->g;
-  d->bw = s->bw;
-  d->ko = key_out(s->sort, s->bw);
+  d->bw = bw_of(s, c);
+  d->ko = key_out(s->sort, d->bw);
   d->offsets = s->sort.offsets;
   d->src_of = s->sort.src_of;
   d->ch = 256u * s->tc.ppt;
@@ -218,7 +234,7 @@ static int zero_grids(mpm_scene *s) {
   for (int32_t *p : i32)
     if (p) HIPCHK(hipMemsetAsync(p, 0, G * 4, s->stream));
   HIPCHK(hipMemsetAsync(s->g.v, 0, G * 16, s->stream));
-  uint8_t *u8[] = {s->bw.ofl, s->bw.act, s->bw.occ};
+  uint8_t *u8[] = {s->bw.ofl, s->bw.act, s->occ_buf[0], s->occ_buf[1]};
   for (uint8_t *p : u8)
     if (p) HIPCHK(hipMemsetAsync(p, 0, s->bw.nbricks, s->stream));
   return MPM_OK;
@@ -235,6 +251,9 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     return failf(MPM_ERR_INVALID, "device %d out of range (%d devices)", c.device, ndev);
   HIPCHK(hipSetDevice(c.device));
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
   const size_t cap = c.capacity ? c.capacity : 1;
   s->G = c.grid_x * c.grid_y * c.grid_z;
   const size_t G = s->G;
@@ -257,17 +276,20 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     bw.nbricks = bw.nbx * bw.nby * bw.nbz;
     s->sort.nkeys = bw.nbricks;
     const uint32_t nit = max_items((uint32_t)cap, bw.nbricks, 4);
-    DALLOC(s->items, (size_t)nit * 16);
-    DALLOC(s->nitems, 4);
+    for (int k = 0; k < 2; k++) {
+      DALLOC(s->items_buf[k], (size_t)nit * 16);
+      DALLOC(s->nitems_buf[k], 4);
+      DALLOC(s->first_buf[k], (size_t)bw.nbricks * 4);
+      DALLOC(s->count_buf[k], (size_t)bw.nbricks * 4);
+      DALLOC(s->occ_buf[k], bw.nbricks);
+      HIPCHK(hipMemsetAsync(s->nitems_buf[k], 0, 4, s->stream));
+      HIPCHK(hipMemsetAsync(s->count_buf[k], 0, (size_t)bw.nbricks * 4, s->stream));
+      HIPCHK(hipMemsetAsync(s->occ_buf[k], 0, bw.nbricks, s->stream));
+    }
     DALLOC(s->brick_of_rank, (size_t)bw.nbricks * 4);
     DALLOC(s->sort.brick_rank, (size_t)bw.nbricks * 4);
-    bw.items = s->items;
-    bw.items_w = s->items;
-    bw.nitems = s->nitems;
     bw.brick_of_rank = s->brick_of_rank;
     bw.brick_rank = s->sort.brick_rank;
-    DALLOC(bw.first, (size_t)bw.nbricks * 4);
-    DALLOC(bw.count, (size_t)bw.nbricks * 4);
     const size_t core = 1000;  // 10^3 ghost cells per item
     DALLOC(bw.gm, (size_t)nit * core * 4);
     DALLOC(bw.gx, (size_t)nit * core * 4);
@@ -279,10 +301,9 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     DALLOC(bw.oz, G * 4);
     DALLOC(bw.ofl, bw.nbricks);
     DALLOC(bw.act, bw.nbricks);
-    DALLOC(bw.occ, bw.nbricks);
     DALLOC(bw.list, (size_t)bw.nbricks * 4);
     DALLOC(bw.nlist, 4);
-    HIPCHK(hipMemsetAsync(bw.count, 0, (size_t)bw.nbricks * 4, s->stream));
+    HIPCHK(hipMemsetAsync(bw.nlist, 0, 4, s->stream));
     DALLOC(s->sort.keys, cap * 4);
     DALLOC(s->sort.rank, cap * 4);
     DALLOC(s->sort.counts, (size_t)s->sort.nkeys * 4);
@@ -290,7 +311,6 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     DALLOC(s->sort.scratch, (size_t)scan_scratch_words(s->sort.nkeys) * 4);
     DALLOC(s->sort.src_of, cap * 4);
     HIPCHK(hipMemsetAsync(s->sort.counts, 0, (size_t)s->sort.nkeys * 4, s->stream));
-    HIPCHK(hipMemsetAsync(s->nitems, 0, 4, s->stream));
     std::vector<uint32_t> rank = morton_tile_rank(bw.nbx, bw.nby, bw.nbz);
     std::vector<uint32_t> inv(rank.size());
     for (size_t i = 0; i < rank.size(); i++) inv[rank[i]] = (uint32_t)i;
@@ -344,6 +364,11 @@ void scene_free(mpm_scene *s) {
   s->stage = nullptr;
   for (int i = 0; i < mpm_scene::kEvPerStep * mpm_scene::kEvSteps; i++)
     if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
+  if (s->side) (void)hipStreamSynchronize(s->side);
+  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+  if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+  if (s->side) (void)hipStreamDestroy(s->side);
+  s->side = nullptr;
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
 }
@@ -513,7 +538,7 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
     bool due = !s->sorted_once || s->since_sort >= period;
     if (due && (!tiled || !s->sorted_once)) {
       PState dst = pstate(s, s->cur ^ 1);
-      launch_sort(P, S, dst, s->sort, s->bw, s->tc, st);
+      launch_sort(P, S, dst, s->sort, bw_of(s, s->cur), s->tc, st);
       s->cur ^= 1;
       S = dst;
       s->since_sort = 0;
@@ -522,33 +547,46 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
       relayout = true;
     }
   }
+  const BrickWork bw = bw_of(s, s->cur);
   if ((rc = mark(1))) return rc;
   if (keep && !tiled) launch_clear_grid(s->g, s->G, st);
   KeyOut ko{};
-  if (relayout) ko = key_out(s->sort, s->bw);
-  if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, s->bw, ko, st);
+  if (relayout) ko = key_out(s->sort, bw);
+  if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, bw, ko, st);
   else launch_p2g_mass_global(P, S, s->g, st);
+  if (relayout) {
+    // the re-sort preparation (offsets, next work items, inverse
+    // permutation) needs only the mass pass: it overlaps the resolves and
+    // P2G momentum on the side stream
+    HIPCHK(hipEventRecord(s->ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+    launch_relayout_prep(P, s->sort, bw, s->tc, s->side);
+    HIPCHK(hipEventRecord(s->ev_join, s->side));
+  }
+  if (tiled) launch_resolve_mass(P, s->g, bw, st);
   if ((rc = mark(2))) return rc;
-  if (tiled) launch_p2g_mom_tiled(P, S, s->g, s->tc, s->bw, st);
+  if (tiled) launch_p2g_mom_tiled(P, S, s->g, s->tc, bw, st);
   else launch_p2g_mom_global(P, S, s->g, st);
   if ((rc = mark(3))) return rc;
-  if (tiled) launch_resolve_update(P, s->g, s->tc, s->bw, s->walls, keep, st);
+  if (tiled) launch_resolve_update(P, s->g, s->tc, bw, s->walls, keep, st);
   else launch_update_dense(P, s->g, s->walls, !keep, st);
   if ((rc = mark(4))) return rc;
-  if (relayout) launch_relayout_prep(P, s->sort, s->bw, s->tc, st);
+  if (relayout) HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
   if ((rc = mark(5))) return rc;
   if (relayout) {
     PState dst = pstate(s, s->cur ^ 1);
-    launch_g2p_tiled(P, S, dst, s->g, s->walls, s->tc, s->bw, s->sort.src_of, st);
+    launch_g2p_tiled(P, S, dst, s->g, s->walls, s->tc, bw, s->sort.src_of, st);
     s->cur ^= 1;
     S = dst;
     s->since_sort = 0;
   } else if (tiled) {
-    launch_g2p_tiled(P, S, S, s->g, s->walls, s->tc, s->bw, nullptr, st);
+    launch_g2p_tiled(P, S, S, s->g, s->walls, s->tc, bw, nullptr, st);
   } else {
     launch_g2p_global(P, S, s->g, s->walls, st);
   }
   if ((rc = mark(6))) return rc;
+  // the resolve list starts every step empty (the re-sorting G2P empties it)
+  if (tiled && !relayout) launch_clear_list(bw, st);
   launch_diffuse(P, S, s->g, s->walls, s->d, st);
   if ((rc = mark(7))) return rc;
   HIPCHK(hipGetLastError());
@@ -853,9 +891,10 @@ int scene_tile_stats(mpm_scene *s, uint64_t out[4]) {
   HIPCHK(hipSetDevice(s->cfg.device));
   HIPCHK(hipStreamSynchronize(s->stream));
   uint32_t nit = 0;
-  HIPCHK(hipMemcpy(&nit, s->nitems, 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&nit, s->nitems_buf[s->cur], 4, hipMemcpyDeviceToHost));
   std::vector<uint4> it(nit);
-  if (nit) HIPCHK(hipMemcpy(it.data(), s->items, (size_t)nit * 16, hipMemcpyDeviceToHost));
+  if (nit)
+    HIPCHK(hipMemcpy(it.data(), s->items_buf[s->cur], (size_t)nit * 16, hipMemcpyDeviceToHost));
   uint64_t mx = 0;
   for (const uint4 &q : it)
     if (q.z > mx) mx = q.z;

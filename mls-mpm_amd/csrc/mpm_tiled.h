@@ -14,11 +14,15 @@ struct TileCfg {
 };
 
 // Work items of the brick-ordered particle arrays.
+// Two sets of work items alternate with the particle buffers: the current
+// set (the particle order the P2G passes read) and the next set, written by
+// a re-sort and read by the re-sorting G2P.
 struct BrickWork {
-  const uint4 *items;            // (brick rank, first particle, count, 0)
-  uint4 *items_w;                // same buffer, written by the sort
-  uint32_t *nitems;              // device count of items
-  uint32_t *first, *count;       // per brick rank: first item and number of items
+  const uint4 *items;            // current: (brick rank, first particle, count, 0)
+  uint4 *items_w;                // next
+  uint32_t *nitems, *nitems_w;   // device counts of items (current, next)
+  uint32_t *first, *count;       // current, per brick rank: first item and number of items
+  uint32_t *first_w, *count_w;   // next
   const uint32_t *brick_of_rank; // brick index of each Morton rank
   const uint32_t *brick_rank;    // Morton rank of each brick index
   int32_t *gm;                   // ghost mass cores (10^3 cells), one per item
@@ -26,7 +30,7 @@ struct BrickWork {
   int32_t *om, *ox, *oy, *oz;    // overflow grids (zero outside flagged bricks)
   uint8_t *ofl;                  // per brick: overflow grid may hold values here
   uint8_t *act;                  // per brick: bit 0 active this step, bit 1 last step
-  uint8_t *occ;                  // per brick: has particles in the current order
+  uint8_t *occ, *occ_w;          // per brick: has particles in the current / next order
   uint32_t *list, *nlist;        // bricks to resolve this step
   uint32_t nbx, nby, nbz, nbricks;
 };
@@ -42,8 +46,13 @@ bool tiled_margin_supported(uint32_t m);
 bool tiled_threads_supported(uint32_t t);
 uint32_t max_items(uint32_t n, uint32_t nbricks, uint32_t ppt);
 KeyOut key_out(const SortBufs &sb, const BrickWork &bw);
+// The resolve list is empty when a step starts: the re-sorting G2P empties
+// it, and a step without a re-sort ends with launch_clear_list.
 void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
                            const BrickWork &bw, const KeyOut &ko, hipStream_t st);
+// The resolve list (bricks to resolve) and the mass resolve.
+void launch_resolve_mass(const Params &P, const GState &g, const BrickWork &bw, hipStream_t st);
+void launch_clear_list(const BrickWork &bw, hipStream_t st);
 void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
                           const BrickWork &bw, hipStream_t st);
 void launch_resolve_update(const Params &P, const GState &g, const TileCfg &tc,

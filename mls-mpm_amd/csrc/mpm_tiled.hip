@@ -59,9 +59,10 @@ __global__ __launch_bounds__(256) void k_g2p_brick(Params P, PState S, PState D,
   g2p_item<M, EXACT>(P, S, D, g, W, bw, src_of, blockIdx.x);
 }
 
-__global__ __launch_bounds__(256) void k_make_items(uint32_t n, BrickWork bw, uint32_t ch,
-                                                    const uint32_t *offsets) {
-  make_items_one(n, bw, ch, offsets, blockIdx.x * blockDim.x + threadIdx.x);
+// Offsets and work items of the new order, one block.
+__global__ __launch_bounds__(1024) void k_scan_items(BrickWork bw, uint32_t chs, uint32_t *counts,
+                                                     uint32_t *offsets) {
+  scan_items_block<8>(bw, chs, counts, offsets);
 }
 
 __global__ __launch_bounds__(256) void k_build_src(uint32_t n, uint32_t nbricks, uint32_t ch,
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(256) void k_sort_gather(uint32_t n, PState src, PSt
 // ---------------------------------------------------------------------------
 // Host launchers.
 static inline uint32_t nblk(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+static inline uint32_t log2_pow2(uint32_t v) { return (uint32_t)__builtin_ctz(v); }
 
 bool tiled_ppt_supported(uint32_t ppt) { return ppt == 4 || ppt == 8 || ppt == 16; }
 bool tiled_margin_supported(uint32_t m) { return m == 1 || m == 2; }
@@ -131,7 +133,11 @@ void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, co
     if (P.unit_mass) k_p2g_mass_brick<2, true><<<grid, tb, 0, st>>>(P, S, bw, ko);
     else k_p2g_mass_brick<2, false><<<grid, tb, 0, st>>>(P, S, bw, ko);
   }
-  k_zero_u32<<<1, 1, 0, st>>>(bw.nlist);
+  (void)g;
+}
+
+void launch_resolve_mass(const Params &P, const GState &g, const BrickWork &bw, hipStream_t st) {
+  if (!P.n) return;
   k_brick_list<<<nblk(bw.nbricks, 256), 256, 0, st>>>(bw);
   k_resolve_mass<<<resolve_grid(bw), 512, 0, st>>>(P, g, bw);
 }
@@ -178,9 +184,7 @@ KeyOut key_out(const SortBufs &sb, const BrickWork &bw) {
 
 static void offsets_items_src(const Params &P, const SortBufs &sb, const BrickWork &bw,
                               const TileCfg &tc, hipStream_t st) {
-  launch_scan(sb.counts, bw.nbricks, false, sb.offsets, sb.scratch, nullptr, st, sb.counts);
-  k_zero_u32<<<1, 1, 0, st>>>(bw.nitems);
-  k_make_items<<<nblk(bw.nbricks, 256), 256, 0, st>>>(P.n, bw, kTB * tc.ppt, sb.offsets);
+  k_scan_items<<<1, 1024, 0, st>>>(bw, log2_pow2(kTB * tc.ppt), sb.counts, sb.offsets);
   k_build_src<<<nblk(P.n, 256), 256, 0, st>>>(P.n, bw.nbricks, kTB * tc.ppt, sb.keys, sb.rank,
                                               sb.offsets, sb.src_of);
 }
@@ -198,6 +202,10 @@ void launch_sort(const Params &P, const PState &src, const PState &dst, const So
 // After a P2G mass pass that recorded keys: offsets, the next step's work
 // items and the inverse permutation for the re-sorting G2P.  Leaves the
 // counts zeroed.
+void launch_clear_list(const BrickWork &bw, hipStream_t st) {
+  k_zero_u32<<<1, 1, 0, st>>>(bw.nlist);
+}
+
 void launch_relayout_prep(const Params &P, const SortBufs &sb, const BrickWork &bw,
                           const TileCfg &tc, hipStream_t st) {
   if (!P.n) return;

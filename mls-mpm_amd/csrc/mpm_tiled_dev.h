@@ -429,8 +429,11 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
   __shared__ float4 lv[V];
   // Re-sort: the item's source slots, staged with the window.
   __shared__ uint32_t lsrc[EXACT ? kTB * kMaxPpt : 1];
-  if (blk >= *bw.nitems) return;
-  const uint4 it = bw.items[blk];
+  // A re-sort writes the next work-item set's order; it also empties the
+  // resolve list for the next step (no resolve runs after G2P).
+  if (EXACT && blk == 0 && threadIdx.x == 0) *bw.nlist = 0u;
+  if (blk >= *(EXACT ? bw.nitems_w : bw.nitems)) return;
+  const uint4 it = (EXACT ? bw.items_w : bw.items)[blk];
   int ibx, iby, ibz;
   brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
   const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
@@ -511,28 +514,85 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
 
 // Work items from the brick offsets: each brick's particle range cut into
 // pieces of at most `ch` particles.  Item order is irrelevant to the result.
-__device__ __forceinline__ void make_items_one(uint32_t n, const BrickWork &bw, uint32_t ch,
-                                               const uint32_t *offsets, uint32_t r) {
-  if (r >= bw.nbricks) return;
-  uint32_t start = offsets[r];
-  uint32_t end = (r + 1 < bw.nbricks) ? offsets[r + 1] : n;
-  uint32_t b = bw.brick_of_rank[r];
-  if (end <= start) {
-    bw.count[r] = 0;
-    bw.first[r] = 0;
-    bw.occ[b] = 0;
-    return;
-  }
-  uint32_t cnt = end - start;
-  uint32_t pieces = (cnt + ch - 1) / ch;
-  uint32_t at = atomicAdd(bw.nitems, pieces);
-  bw.first[r] = at;
-  bw.count[r] = pieces;
-  bw.occ[b] = 1;
+// Work items of a brick (index b, rank r): its range cut into pieces of at
+// most ch = 1 << chs particles.
+__device__ __forceinline__ void write_items(const BrickWork &bw, uint32_t r, uint32_t b,
+                                            uint32_t start, uint32_t cnt, uint32_t chs,
+                                            uint32_t item0) {
+  const uint32_t ch = 1u << chs;
+  const uint32_t pieces = (cnt + ch - 1) >> chs;
+  bw.first_w[r] = cnt ? item0 : 0u;
+  bw.count_w[r] = pieces;
+  bw.occ_w[b] = cnt ? 1 : 0;
   for (uint32_t p = 0; p < pieces; p++) {
-    uint32_t c = cnt - p * ch;
-    bw.items_w[at + p] = make_uint4(r, start + p * ch, c < ch ? c : ch, 0u);
+    const uint32_t c = cnt - p * ch;
+    bw.items_w[item0 + p] = make_uint4(r, start + p * ch, c < ch ? c : ch, 0u);
   }
+}
+
+// One block: exclusive scans of the per-brick particle counts (the new
+// brick offsets) and of the per-brick work-item counts, the work items of
+// the new order (numbered in brick-rank order) and their number; the counts
+// are cleared for the next step.  Writes the next work-item set.  A round
+// covers K * blockDim bricks, row k of it bricks base + k * blockDim + t, so
+// every load and store of a row is coalesced; the rows are scanned in turn.
+// The block size is a multiple of 64, at most 1024.
+template <int K>
+__device__ __forceinline__ void scan_items_block(const BrickWork &bw, uint32_t chs,
+                                                 uint32_t *counts, uint32_t *offsets) {
+  __shared__ uint32_t wp[16], wi[16];
+  const uint32_t ch = 1u << chs;
+  const uint32_t t = threadIdx.x, nt = blockDim.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t nw = nt >> 6, nb = bw.nbricks;
+  uint32_t carry_p = 0, carry_i = 0;  // running totals, the same in every thread
+  for (uint32_t base = 0; base < nb; base += nt * K) {
+    uint32_t c[K], bk[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const uint32_t r = base + k * nt + t;
+      c[k] = r < nb ? counts[r] : 0u;
+      bk[k] = r < nb ? bw.brick_of_rank[r] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const uint32_t r = base + k * nt + t;
+      const uint32_t pc = (c[k] + ch - 1) >> chs;
+      uint32_t ip = c[k], ii = pc;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t vp = __shfl_up(ip, off, 64), vi = __shfl_up(ii, off, 64);
+        if (lane >= (uint32_t)off) {
+          ip += vp;
+          ii += vi;
+        }
+      }
+      if (lane == 63u) {
+        wp[wave] = ip;
+        wi[wave] = ii;
+      }
+      __syncthreads();
+      uint32_t ep = carry_p, ei = carry_i, tp = 0, ti = 0;
+      for (uint32_t w = 0; w < nw; w++) {
+        if (w < wave) {
+          ep += wp[w];
+          ei += wi[w];
+        }
+        tp += wp[w];
+        ti += wi[w];
+      }
+      __syncthreads();  // the wave sums are read
+      ep += ip - c[k];
+      ei += ii - pc;
+      if (r < nb) {
+        offsets[r] = ep;
+        counts[r] = 0u;
+        write_items(bw, r, bk[k], ep, c[k], chs, ei);
+      }
+      carry_p += tp;
+      carry_i += ti;
+    }
+  }
+  if (t == 0) *bw.nitems_w = carry_i;
 }
 
 
