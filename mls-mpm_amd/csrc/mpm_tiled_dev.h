@@ -110,12 +110,25 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
       int dx = bx - ibx + 1, dy = by - iby + 1, dz = bz - ibz + 1;
       uint32_t dest = bw.brick_rank[brick_index(bw, bx, by, bz)];
       ko.keys[s] = dest;
-      if (dx >= 0 && dx < 3 && dy >= 0 && dy < 3 && dz >= 0 && dz < 3) {
-        uint32_t nb = (uint32_t)((dz * 3 + dy) * 3 + dx);
-        ko.rank[s] = (nb << 26) | atomicAdd(&lcnt[nb], 1u);
+      const bool near = dx >= 0 && dx < 3 && dy >= 0 && dy < 3 && dz >= 0 && dz < 3;
+      const uint32_t nb = near ? (uint32_t)((dz * 3 + dy) * 3 + dx) : kFar;
+      // Most particles stay in their own brick (nb 13): those of a
+      // wavefront take consecutive ranks with one LDS atomic.
+      const uint64_t own = __ballot(nb == 13u);
+      uint32_t r;
+      if (nb == 13u) {
+        const uint32_t lane = __lane_id();
+        const int leader = __ffsll((long long)own) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&lcnt[13], (uint32_t)__popcll(own));
+        base = __shfl(base, leader, 64);
+        r = base + (uint32_t)__popcll(own & ((1ull << lane) - 1ull));
+      } else if (near) {
+        r = atomicAdd(&lcnt[nb], 1u);
       } else {
-        ko.rank[s] = (kFar << 26) | atomicAdd(&ko.counts[dest], 1u);
+        r = atomicAdd(&ko.counts[dest], 1u);
       }
+      ko.rank[s] = (nb << 26) | r;
     }
     float pm = UNIT ? 1.0f : S.at(PM, s);
     WinPos wp = win_pos<M>(pw, ox, oy, oz);
