@@ -4,9 +4,13 @@
 Workload (default): 3-D dam break on a 256x128x128 grid, 4,199,136 FP32
 particles on a 0.5-cell lattice (x in [2,104), y in [2,43.5), z in [2,126),
 seeded jitter +/-0.1 cell), i32 fixed-point grids, reference physics
-constants, particles re-sorted by cell every 8 steps, no diffuse particles.
-A "step" is one whole MLS-MPM step (P2G mass, P2G momentum, grid update,
-G2P, plus the periodic sort) over all particles, state resident in HBM.
+constants, no diffuse particles.  Particles are re-laid out by brick inside
+every G2P (--sort-period 1, the fastest period; --sort-period 8 re-sorts every
+8 steps as the workload is specified, with identical grids).  A "step" is one
+whole MLS-MPM step (P2G mass, P2G momentum, grid update, G2P with the
+re-layout) over all particles, state resident in HBM.  The timed run is
+--steps steps (default 1000, the workload's run length) from the initial
+state: --warmup steps capture the step graphs, then the scene is reset.
 
 Prints one JSON line:
   value            particle-steps/s over the timed region (wall clock,
@@ -14,7 +18,8 @@ Prints one JSON line:
   roofline         the dominant kernel's algorithmic bytes per launch over its
                    mean launch time, from HIP events recorded on the scene's
                    stream around that kernel alone in a second timed region of
-                   the same length, against the 8 TB/s HBM peak; `traffic` is
+                   the same steps from the same reset, against the 8 TB/s HBM
+                   peak (active cells: the mean of ten samples over that run); `traffic` is
                    the HBM bytes per launch that rocprofv3 FETCH_SIZE +
                    WRITE_SIZE read for that kernel (profiles/traffic.json),
                    scaled by the same counters' reading for a plain copy
@@ -254,9 +259,13 @@ def serve_bench(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--workload", default="dam-large", choices=["dam-large", "dam-small", "two-block"])
+    ap.add_argument("--diffuse", type=int, default=-1,
+                    help="diffuse slots (default: 2,097,152 for two-block, else 0)")
+    ap.add_argument("--lifetime-mode", type=int, default=0,
+                    help="foam lifetime: 0 reference, 1 uniform [0.5, 3] s")
     ap.add_argument("--sort-period", type=int, default=1)
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--tile-ppt", type=int, default=0)
@@ -302,17 +311,22 @@ def main() -> int:
 
     grid, pos, vel, desc = workload(args.workload, args.seed)
     n = len(pos)
+    D = args.diffuse if args.diffuse >= 0 else (2_097_152 if args.workload == "two-block" else 0)
     cfg = mlsmpm.default_config(*grid, n, sort_period=args.sort_period, flags=args.flags, device=0,
                                 tile_ppt=args.tile_ppt, tile_margin=args.tile_margin,
-                                tile_threads=args.tile_threads, g2p_threads=args.g2p_threads)
+                                tile_threads=args.tile_threads, g2p_threads=args.g2p_threads,
+                                diffuse_capacity=D, lifetime_mode=args.lifetime_mode)
     s = mlsmpm.Scene(cfg)
     s.reset(pos, vel)
     print(f"[bench] {desc}: {n} particles, warmup {args.warmup}", file=sys.stderr, flush=True)
+    # Warm-up captures the step graphs; each region then runs the workload's
+    # steps from the initial state (a reset to the same particle count keeps
+    # the graphs).
     s.step(args.warmup)
     s.sync()
-    g0 = int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"]))
 
     # Region 1: throughput, wall clock only.
+    s.reset(pos, vel)
     s.enable_timing(False)
     s.sync()
     t0 = time.perf_counter()
@@ -321,19 +335,34 @@ def main() -> int:
     t1 = time.perf_counter()
     elapsed = t1 - t0
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
-    # Region 2: the same number of steps with HIP events around every stage.
+    # Region 2: the same steps from the same start with HIP events around
+    # every stage; active cells are counted every tenth of the run.
     stage = np.zeros(len(STAGES), np.float32)
     step_ms = np.zeros(0, np.float32)
+    events_wall = None
+    g_samples = []
     if not args.no_events:
+        s.reset(pos, vel)
         s.enable_timing(True)
-        s.step(args.steps)
+        chunk = max(1, args.steps // 10)
+        done = 0
+        t_ev = 0.0
+        while done < args.steps:
+            k = min(chunk, args.steps - done)
+            t2 = time.perf_counter()
+            s.step(k)
+            s.sync()
+            t_ev += time.perf_counter() - t2
+            done += k
+            g_samples.append(int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"])))
+        events_wall = t_ev / args.steps * 1e3
         stage = s.stage_times()
         step_ms = s.step_times()
         s.enable_timing(False)
-    g1 = int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"]))
+    else:
+        g_samples.append(int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"])))
     tstats = s.tile_stats()
-
-    g_act = 0.5 * (g0 + g1)
+    g_act = float(np.mean(g_samples))
     ms = {k: float(v) for k, v in zip(STAGES, stage)}
     # dominant kernel and its roofline
     dom = max(("g2p", "p2g_mom", "update"), key=lambda k: ms[k])
@@ -348,7 +377,10 @@ def main() -> int:
         ent = tr.get(args.workload, {}).get(dom)
         if ent:
             traffic = ent.get("bytes_per_launch")
-    step_alg = 156 * n + 76 * g_act
+    d_alive = s.diffuse_alive() if D else 0
+    # diffuse term: density read per particle, alive flag per slot, 72 B per
+    # alive record (spawn and kill traffic left out)
+    step_alg = 156 * n + 76 * g_act + (4 * n + 4 * D + 72 * d_alive if D else 0)
     step_ms_mean = elapsed / args.steps * 1e3
     line = {
         "metric": "particle-steps/s + % HBM roofline; batched/served vs plain; p99 step latency",
@@ -369,7 +401,9 @@ def main() -> int:
             "grid": list(grid),
             "grid_dtype": "i32 fixed point (x1e5)",
             "sort_period": args.sort_period,
-            "diffuse": 0,
+            "diffuse": D,
+            "diffuse_alive_end": d_alive,
+            "lifetime_mode": args.lifetime_mode,
             "active_cells_mean": g_act,
             "parallelism": "single GPU",
         },
@@ -388,6 +422,7 @@ def main() -> int:
         "stage_ms": ms,
         "tile_stats": tstats,
         "events_ms_per_step": float(ms["step"]),
+        "events_region_wall_ms_per_step": events_wall,
         "step_latency_ms": {
             "p50": float(np.percentile(step_ms, 50)) if len(step_ms) else None,
             "p99": float(np.percentile(step_ms, 99)) if len(step_ms) else None,

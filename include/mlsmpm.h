@@ -63,7 +63,10 @@ typedef struct mpm_config {
     uint32_t tile_margin;  /* P2G LDS window drift margin in cells (1/2); 0 = 2 */
     uint32_t tile_threads; /* P2G threads per work item (256); 0 = 256 */
     uint32_t g2p_threads;  /* G2P threads per work item (256); 0 = 256 */
-    uint32_t reserved[2];
+    uint32_t lifetime_mode; /* foam lifetime: 0 = the reference's LIFETIME_FOAM * min(1,
+                               score / KE_THRESHOLD); 1 = uniform in [0.5, 3] s, seeded
+                               (0.5 + 2.5 * diffuse_rand(spawn seed + 4)) */
+    uint32_t reserved[1];
 } mpm_config;
 
 typedef struct mpm_scene mpm_scene;

@@ -33,6 +33,7 @@ struct Params {
   uint32_t dmax;
   uint32_t unit_mass;  // 1 when every particle mass is exactly 1.0f
   uint32_t nwalls;
+  uint32_t lifetime_mode;  // 0: reference foam lifetime, 1: uniform [0.5, 3] s
 };
 
 // One moving wall, 20 floats (shaders.js:123-129).

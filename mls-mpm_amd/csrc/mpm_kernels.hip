@@ -235,8 +235,10 @@ __global__ __launch_bounds__(256) void k_gen_apply(Params P, PState S, GState g,
     float jy = ((diffuse_rand(e.rand_seed + 2u) - 0.5f) * jitter) * e.speed;
     float jz = ((diffuse_rand(e.rand_seed + 3u) - 0.5f) * jitter) * e.speed;
     uint32_t ptype = (e.density < P.density_spray_max) ? 0u : 1u;
-    float lifetime = (ptype == 1u) ? P.lifetime_foam * fminf(1.0f, e.score / P.ke_threshold)
-                                   : 999.0f;
+    float lifetime = 999.0f;
+    if (ptype == 1u)
+      lifetime = P.lifetime_mode ? 0.5f + 2.5f * diffuse_rand(e.rand_seed + 4u)
+                                 : P.lifetime_foam * fminf(1.0f, e.score / P.ke_threshold);
     DRec rec;
     rec.pos = make_float4(S.at(PX, s), S.at(PY, s), S.at(PZ, s), 0.0f);
     rec.vel = make_float4(S.at(VX, s) + jx, S.at(VY, s) + jy, S.at(VZ, s) + jz, 0.0f);

@@ -49,6 +49,9 @@ int validate_config(const mpm_config &c) {
   if ((c.tile_threads && !tiled_threads_supported(c.tile_threads)) ||
       (c.g2p_threads && !tiled_threads_supported(c.g2p_threads)))
     return failf(MPM_ERR_INVALID, "tile_threads / g2p_threads: supported 256");
+  if (c.lifetime_mode > 1)
+    return failf(MPM_ERR_INVALID, "lifetime_mode %u: 0 (reference) or 1 (uniform)",
+                 c.lifetime_mode);
   if (c.eos_power != 1.0f)
     return failf(MPM_ERR_UNSUPPORTED, "eos_power %g: only 1 (the reference default) is built",
                  (double)c.eos_power);
@@ -141,6 +144,7 @@ Params scene_params(const mpm_scene *s) {
   P.dmax = c.diffuse_capacity;
   P.unit_mass = (uint32_t)s->unit_mass;
   P.nwalls = s->nwalls;
+  P.lifetime_mode = c.lifetime_mode;
   return P;
 }
 
@@ -421,7 +425,11 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
   }
   HIPCHK(hipSetDevice(c.device));
   HIPCHK(hipStreamSynchronize(s->stream));
-  scene_drop_graph(s);
+  // A captured step depends on the host state only through the launch
+  // parameters (particle count, unit mass; walls and constants drop the
+  // graphs where they change) and its start state (cur, since_sort), so a
+  // reset to the same count and mass kind keeps the graphs.
+  if (n != s->n || unit != s->unit_mass) scene_drop_graph(s);
   s->version++;
   s->n = n;
   s->cur = 0;

@@ -133,7 +133,7 @@ static const field_t kFields[] = {
     FF(wall_stiffness), FF(lookahead_k), FF(ke_threshold), FF(pressure_threshold),
     FF(crest_dot_threshold), FF(spawn_rate_k), FF(lifetime_foam), FF(buoyancy), FF(drag),
     FF(density_spray_max), FF(density_bubble_min), FU(seed), FU(tile_ppt), FU(tile_margin),
-    FU(tile_threads), FU(g2p_threads),
+    FU(tile_threads), FU(g2p_threads), FU(lifetime_mode),
 };
 
 static napi_value config_to_js(napi_env env, const mpm_config *c) {

@@ -56,7 +56,8 @@ class Config(C.Structure):
         ("lifetime_foam", C.c_float), ("buoyancy", C.c_float), ("drag", C.c_float),
         ("density_spray_max", C.c_float), ("density_bubble_min", C.c_float),
         ("seed", C.c_uint32), ("tile_ppt", C.c_uint32), ("tile_margin", C.c_uint32),
-        ("tile_threads", C.c_uint32), ("g2p_threads", C.c_uint32), ("reserved", C.c_uint32 * 2),
+        ("tile_threads", C.c_uint32), ("g2p_threads", C.c_uint32),
+        ("lifetime_mode", C.c_uint32), ("reserved", C.c_uint32 * 1),
     ]
 
     def copy(self) -> "Config":

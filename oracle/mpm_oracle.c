@@ -518,9 +518,11 @@ void orc_diffuse_generate(const orc_params *P, const orc_gen *gen, const real *p
         real jz = ((drand(randSeed + 3u) - K(0.5)) * jitter) * speed;
         uint32_t ptype = DIFFUSE_FOAM;
         if (density < (real)gen->density_spray_max) ptype = DIFFUSE_SPRAY;
-        real lifetime = (ptype == DIFFUSE_FOAM)
-                            ? (real)gen->lifetime_foam * MINR(K(1.0), score / keth)
-                            : K(999.0);
+        real lifetime = K(999.0);
+        if (ptype == DIFFUSE_FOAM)
+            lifetime = gen->lifetime_mode
+                           ? K(0.5) + K(2.5) * drand(randSeed + 4u)
+                           : (real)gen->lifetime_foam * MINR(K(1.0), score / keth);
         orc_drec r;
         r.pos[0] = px; r.pos[1] = py; r.pos[2] = pz; r.pos[3] = K(0.0);
         r.vel[0] = vx + jx; r.vel[1] = vy + jy; r.vel[2] = vz + jz; r.vel[3] = K(0.0);

@@ -59,6 +59,7 @@ typedef struct {
     float ke_threshold, spawn_rate_k, density_spray_max, density_bubble_min;
     float lifetime_foam, pressure_threshold, crest_dot_threshold;
     uint32_t seed;
+    uint32_t lifetime_mode; /* 0: reference; 1: uniform [0.5, 3] (extension, not in the reference) */
 } orc_gen;
 
 /* AdvectParams (shaders.js:872-877; diffuse-gpu.js:70-76). */

@@ -51,6 +51,7 @@ class Gen(C.Structure):
         ("density_spray_max", C.c_float), ("density_bubble_min", C.c_float),
         ("lifetime_foam", C.c_float), ("pressure_threshold", C.c_float),
         ("crest_dot_threshold", C.c_float), ("seed", C.c_uint32),
+        ("lifetime_mode", C.c_uint32),
     ]
 
 

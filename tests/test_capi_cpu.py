@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_size_matches_header():
-    # 8 u32 + 22 f32 + seed + 4 tile fields + 2 reserved = 37 words
+    # 8 u32 + 22 f32 + seed + 4 tile fields + lifetime_mode + 1 reserved = 37 words
     assert C.sizeof(capi.Config) == 37 * 4
 
 

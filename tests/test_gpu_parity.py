@@ -127,6 +127,23 @@ def test_diffuse_every_step(mode):
         check_equal(s, o, f"step {step}", diffuse=True)
 
 
+@pytest.mark.parametrize("mode", ["direct", "sorted1"])
+def test_diffuse_uniform_lifetime_mode(mode):
+    """lifetime_mode 1 (foam lifetime uniform in [0.5, 3] s, seeded): pool
+    bit-exact against the CPU restatement in the same mode."""
+    grid = (32, 32, 32)
+    pos = scenes.lattice_block((2, 2, 2), (14, 22, 14), 0.1, np.random.default_rng(4))
+    s, o = make_pair(grid, pos, D=20000, lifetime_mode=1, **MODES[mode])
+    o.gen.lifetime_mode = 1
+    for step in range(12):
+        s.step(1)
+        o.step(1)
+        check_equal(s, o, f"step {step}", diffuse=True)
+    rec, _, _ = s.read_diffuse()
+    foam = rec[(rec["alive"] == 1) & (rec["ptype"] == 1)]
+    assert len(foam) > 0 and np.all(foam["lifetime"] <= 3.0)
+
+
 @pytest.mark.parametrize("mode", ["direct", "sorted8"])
 def test_diffuse_pool_saturation(mode):
     """A pool far smaller than the spawn rate: lowest particle index wins."""
@@ -235,6 +252,38 @@ def test_graph_replay_matches_eager_and_oracle(period, timed):
     np.testing.assert_array_equal(flg, fle)
     g.close()
     e.close()
+
+
+@pytest.mark.parametrize("period", [1, 8])
+def test_reset_keeps_graphs_and_bits(period):
+    """A reset to the same particle count keeps the captured graphs: the run
+    after the reset must equal a fresh scene's (and the oracle's), with the
+    diffuse pool on; a reset to another count drops them and still matches."""
+    grid = (40, 32, 36)
+    pos = scenes.lattice_block((2, 2, 2), (22, 22, 20), 0.1, np.random.default_rng(12))
+    D = 20000
+    kw = dict(flags=KEEP, sort_period=period, tile_ppt=4)
+    g, _ = make_pair(grid, pos, D=D, **kw)
+    g.step(40)
+    g.reset(pos)
+    g.step(40)
+    f, o = make_pair(grid, pos, D=D, **kw)
+    f.step(40)
+    o.step(40)
+    check_equal(g, o, "reset-and-replay vs oracle after 40 steps", diffuse=True)
+    pg, pf = g.read_particles(), f.read_particles()
+    for k in pg:
+        np.testing.assert_array_equal(pg[k], pf[k], err_msg=f"reset-and-replay vs fresh {k}")
+    # a smaller particle set: launch sizes change, the graphs are re-captured
+    sub = pos[: len(pos) // 2 + 7]
+    g.reset(sub)
+    g.step(40)
+    f2, o2 = make_pair(grid, sub, D=D, **kw)
+    o2.step(40)
+    check_equal(g, o2, "after reset to a smaller count", diffuse=True)
+    g.close()
+    f.close()
+    f2.close()
 
 
 def wall_block(count):

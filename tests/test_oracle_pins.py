@@ -342,6 +342,28 @@ def test_generate_spawns_spray_and_foam():
     assert np.all(np.abs(foam["vel"][:3] - [0, 4.0, 0]) <= 0.15 * 4.0 + 1e-6)
 
 
+def test_uniform_lifetime_mode_draws_foam_lifetimes_in_range():
+    """lifetime_mode 1 (extension: foam lifetime uniform in [0.5, 3] s) draws
+    0.5 + 2.5 * diffuse_rand(spawn seed + 4); spray keeps 999."""
+    s = _pool_scene()
+    pos = np.array([[8.5, 8.5, 8.5], [6.5, 9.5, 7.5], [7.5, 7.5, 9.5], [9.5, 7.5, 7.5]])
+    s.reset(pos, vel=np.tile([0.0, 4.0, 0.0], (4, 1)))
+    s.density[:] = [1.0, 3.0, 3.0, 3.0]
+    s.gen.seed = 5
+    s.gen.lifetime_mode = 1
+    s.k_generate()
+    alive = s.pool[s.pool["alive"] == 1]
+    foam = alive[alive["ptype"] == 1]
+    spray = alive[alive["ptype"] == 0]
+    assert len(foam) == 3 and len(spray) == 1
+    assert spray["lifetime"][0] == 999.0
+    assert np.all((foam["lifetime"] >= 0.5) & (foam["lifetime"] <= 3.0))
+    assert len(set(foam["lifetime"].tolist())) == 3
+    # each is 0.5 + 2.5 * rand(p * 9781 + seed + 4) of its spawner p
+    want = sorted(F32(0.5) + F32(2.5) * F32(O.diffuse_rand(p * 9781 + 5 + 4)) for p in (1, 2, 3))
+    assert sorted(foam["lifetime"].tolist()) == [float(w) for w in want]
+
+
 def test_generate_respects_density_window_and_pool_exhaustion():
     s = _pool_scene(D=1)
     s.reset(np.array([[8.5, 8.5, 8.5]] * 3), vel=np.array([[3.0, 0, 0]] * 3))
