@@ -85,25 +85,25 @@ __global__ __launch_bounds__(256) void k_batch_brick_list(const SceneDev *__rest
   }
 }
 
-__global__ __launch_bounds__(512) void k_batch_resolve_mass(const SceneDev *__restrict__ sd,
+__global__ __launch_bounds__(kResolveThreads) void k_batch_resolve_mass(const SceneDev *__restrict__ sd,
                                                             const uint32_t *__restrict__ pre,
                                                             uint32_t ns, uint32_t total) {
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     const uint32_t si = find_scene(pre, ns, u);
     const SceneDev &d = sd[si];
     const uint32_t i = u - pre[si];
-    if (i < *d.bw.nlist) resolve_mass_entry(d.P, d.g, d.bw, i);
+    if (i < *d.bw.nlist) resolve_mass_entry<kResolveCells>(d.P, d.g, d.bw, i);
   }
 }
 
-__global__ __launch_bounds__(512) void k_batch_resolve_update(const SceneDev *__restrict__ sd,
+__global__ __launch_bounds__(kResolveThreads) void k_batch_resolve_update(const SceneDev *__restrict__ sd,
                                                               const uint32_t *__restrict__ pre,
                                                               uint32_t ns, uint32_t total) {
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     const uint32_t si = find_scene(pre, ns, u);
     const SceneDev &d = sd[si];
     const uint32_t i = u - pre[si];
-    if (i < *d.bw.nlist) resolve_update_entry(d.P, d.g, d.bw, d.W, d.keep, i);
+    if (i < *d.bw.nlist) resolve_update_entry<kResolveCells>(d.P, d.g, d.bw, d.W, d.keep, i);
   }
 }
 
@@ -159,10 +159,10 @@ int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &gr
   e = hipEventRecord(join, side);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step join");
   k_batch_brick_list<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks);
-  k_batch_resolve_mass<<<gr, 512, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
+  k_batch_resolve_mass<<<gr, kResolveThreads, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
   MPM_BATCH_ITEMS(k_batch_mom);
 #undef MPM_BATCH_ITEMS
-  k_batch_resolve_update<<<gr, 512, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
+  k_batch_resolve_update<<<gr, kResolveThreads, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
   e = hipStreamWaitEvent(st, join, 0);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step wait");
   k_batch_g2p<<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);

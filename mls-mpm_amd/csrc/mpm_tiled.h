@@ -21,7 +21,7 @@ struct BrickWork {
   const uint4 *items;            // current: (brick rank, first particle, count, 0)
   uint4 *items_w;                // next
   uint32_t *nitems, *nitems_w;   // device counts of items (current, next)
-  uint32_t *first, *count;       // current, per brick rank: first item and number of items
+  uint32_t *first, *count;       // current, per brick index: first item and number of items
   uint32_t *first_w, *count_w;   // next
   const uint32_t *brick_of_rank; // brick index of each Morton rank
   const uint32_t *brick_rank;    // Morton rank of each brick index

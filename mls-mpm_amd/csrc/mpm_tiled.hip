@@ -71,26 +71,50 @@ __global__ __launch_bounds__(256) void k_build_src(uint32_t n, uint32_t nbricks,
   build_src_one(n, nbricks, ch, keys, rank, offsets, src_of, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__global__ __launch_bounds__(512) void k_resolve_mass(Params P, GState g, BrickWork bw) {
+__global__ __launch_bounds__(kResolveThreads) void k_resolve_mass(Params P, GState g,
+                                                                  BrickWork bw) {
   const uint32_t nl = *bw.nlist;
-  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) resolve_mass_entry(P, g, bw, i);
+  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x)
+    resolve_mass_entry<kResolveCells>(P, g, bw, i);
 }
 
-__global__ __launch_bounds__(512) void k_resolve_update(Params P, GState g, BrickWork bw, Walls W,
-                                                        int keep) {
+__global__ __launch_bounds__(kResolveThreads) void k_resolve_update(Params P, GState g,
+                                                                    BrickWork bw, Walls W,
+                                                                    int keep) {
   const uint32_t nl = *bw.nlist;
-  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) resolve_update_entry(P, g, bw, W, keep, i);
+  for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x)
+    resolve_update_entry<kResolveCells>(P, g, bw, W, keep, i);
 }
 
-// First-step keys: brick and rank of every particle by global atomics.
+// First-step keys: brick and rank of every particle.  Neighbouring slots
+// mostly share a brick (a reset fills bricks row by row), so the wavefront
+// takes one global atomic per distinct brick among its lanes and hands out
+// the ranks by popcount, instead of 64 atomics on one counter.
 __global__ __launch_bounds__(256) void k_sort_keys(Params P, PState S, BrickWork bw, KeyOut ko) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= P.n) return;
-  int bx, by, bz;
-  cell_brick(P, S.at(PX, s), S.at(PY, s), S.at(PZ, s), bx, by, bz);
-  uint32_t dest = bw.brick_rank[brick_index(bw, bx, by, bz)];
-  ko.keys[s] = dest;
-  ko.rank[s] = atomicAdd(&ko.counts[dest], 1u);
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = s < P.n;
+  uint32_t dest = 0u;
+  if (live) {
+    int bx, by, bz;
+    cell_brick(P, S.at(PX, s), S.at(PY, s), S.at(PZ, s), bx, by, bz);
+    dest = bw.brick_rank[brick_index(bw, bx, by, bz)];
+    ko.keys[s] = dest;
+  }
+  // every lane stays in the loop: the shuffles read all 64 lanes
+  const uint32_t lane = __lane_id();
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint64_t pending = __ballot(live);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t ld = __shfl(dest, leader, 64);
+    const bool mine = live && dest == ld && ((pending >> lane) & 1ull);
+    const uint64_t same = __ballot(mine);
+    uint32_t base = 0u;
+    if (lane == (uint32_t)leader) base = atomicAdd(&ko.counts[ld], (uint32_t)__popcll(same));
+    base = __shfl(base, leader, 64);
+    if (mine) ko.rank[s] = base + (uint32_t)__popcll(same & below);
+    pending &= ~same;
+  }
 }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
@@ -139,7 +163,7 @@ void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, co
 void launch_resolve_mass(const Params &P, const GState &g, const BrickWork &bw, hipStream_t st) {
   if (!P.n) return;
   k_brick_list<<<nblk(bw.nbricks, 256), 256, 0, st>>>(bw);
-  k_resolve_mass<<<resolve_grid(bw), 512, 0, st>>>(P, g, bw);
+  k_resolve_mass<<<resolve_grid(bw), kResolveThreads, 0, st>>>(P, g, bw);
 }
 
 void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
@@ -158,7 +182,7 @@ void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, con
 void launch_resolve_update(const Params &P, const GState &g, const TileCfg &tc,
                            const BrickWork &bw, const Walls &W, bool keep, hipStream_t st) {
   (void)tc;
-  k_resolve_update<<<resolve_grid(bw), 512, 0, st>>>(P, g, bw, W, keep ? 1 : 0);
+  k_resolve_update<<<resolve_grid(bw), kResolveThreads, 0, st>>>(P, g, bw, W, keep ? 1 : 0);
 }
 
 // G2P over the current work items.  Re-sort (src_of set): items of the new

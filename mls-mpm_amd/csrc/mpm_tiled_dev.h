@@ -14,6 +14,9 @@ constexpr int kTB = 256;    // row width of an item (slots per row)
 constexpr int kMaxPpt = 16; // largest particles-per-thread of a work item
 constexpr int kCore = 10;   // brick + 1-cell halo per axis
 constexpr int kCoreV = kCore * kCore * kCore;
+// Resolve blocks: cells per thread and threads per block (one 8^3 brick).
+constexpr int kResolveCells = 2;
+constexpr int kResolveThreads = 512 / kResolveCells;
 constexpr uint32_t kFar = 31u;  // rank tag: particle left the 27-brick neighbourhood
 
 // Brick of a cell, clamped into the grid.
@@ -289,7 +292,7 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
 // ---------------------------------------------------------------------------
 // Resolve.  list holds the bricks whose cells can be non-zero this step or
 // were non-zero last step: bricks next to an occupied brick, flagged bricks
-// and bricks active last step.  One block (512 threads, one per cell) per
+// and bricks active last step.  One block (256 threads, two cells each) per
 // listed brick, persistent over the list.
 __device__ __forceinline__ void brick_list_one(const BrickWork &bw, uint32_t b) {
   if (b >= bw.nbricks) return;
@@ -309,6 +312,8 @@ struct NbrItems {
   uint32_t first[27], count[27];
 };
 
+// (first item, item count) of the 27 bricks around b, by brick index (zero
+// outside the grid); returns whether any of them has items.
 __device__ __forceinline__ bool load_neighbours(const BrickWork &bw, uint32_t b, NbrItems &nb,
                                                 int t) {
   __shared__ int any;
@@ -323,9 +328,9 @@ __device__ __forceinline__ bool load_neighbours(const BrickWork &bw, uint32_t b,
     uint32_t cnt = 0, first = 0;
     if (bx >= 0 && by >= 0 && bz >= 0 && bx < (int)bw.nbx && by < (int)bw.nby &&
         bz < (int)bw.nbz) {
-      uint32_t r = bw.brick_rank[brick_index(bw, bx, by, bz)];
-      cnt = bw.count[r];
-      first = bw.first[r];
+      const uint32_t nbi = brick_index(bw, bx, by, bz);
+      cnt = bw.count[nbi];
+      first = bw.first[nbi];
     }
     nb.first[t] = first;
     nb.count[t] = cnt;
@@ -335,48 +340,109 @@ __device__ __forceinline__ bool load_neighbours(const BrickWork &bw, uint32_t b,
   return any != 0;
 }
 
-// Calls f(ghost index) for every ghost core that covers cell (lx,ly,lz) of
-// the brick (local coords 0..7): its own brick's items, and the neighbour
-// bricks' items whose core reaches one cell into this brick.
-template <class F>
-__device__ __forceinline__ void for_covering_items(const NbrItems &nb, int lx, int ly, int lz,
-                                                   F f) {
-  const int x0 = lx == 0 ? 0 : 1, x1 = lx == 7 ? 2 : 1;
-  const int y0 = ly == 0 ? 0 : 1, y1 = ly == 7 ? 2 : 1;
-  const int z0 = lz == 0 ? 0 : 1, z1 = lz == 7 ? 2 : 1;
-  for (int dz = z0; dz <= z1; dz++)
-    for (int dy = y0; dy <= y1; dy++)
-      for (int dx = x0; dx <= x1; dx++) {
-        int n = (dz * 3 + dy) * 3 + dx;
-        if (!nb.count[n]) continue;
-        // core coordinates of this cell in brick n (core origin = brick n - 1)
-        int cx = lx - (dx - 1) * 8 + 1, cy = ly - (dy - 1) * 8 + 1, cz = lz - (dz - 1) * 8 + 1;
-        int li = (cz * kCore + cy) * kCore + cx;
-        for (uint32_t k = 0; k < nb.count[n]; k++) f((size_t)(nb.first[n] + k) * kCoreV + li);
-      }
+// The ghost arrays a resolve sums (mass: 1, momentum: 3).
+template <int R>
+struct GhostSrc {
+  const int32_t *p[R];
+};
+
+// Adds the ghost values of items [f, f + cn) at core offsets li[0..C-1] into
+// acc.  Four items per round: the item index is clamped into the range and
+// the excess masked to zero, so every load is in bounds and a thread keeps
+// 4 * C * R loads in flight.  Integer sums: the order is free.
+template <int R, int C>
+__device__ __forceinline__ void add_items(const GhostSrc<R> &gs, uint32_t f, uint32_t cn,
+                                          const int (&li)[C], uint32_t (&acc)[C][R]) {
+  for (uint32_t k = 0; k < cn; k += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t kk = k + u < cn ? k + u : cn - 1u;
+      const uint32_t keep = k + u < cn ? ~0u : 0u;
+      const size_t base = (size_t)(f + kk) * kCoreV;
+#pragma unroll
+      for (int j = 0; j < C; j++)
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[j][r] += (uint32_t)gs.p[r][base + li[j]] & keep;
+    }
+  }
 }
 
-// One listed brick per block (512 threads, one per cell); block-uniform.
+// Ghost sums of the C cells c_j = t + j * (512 / C) of the brick: the
+// brick's own items cover every cell; the items of a neighbour brick reach
+// one cell into it (the cells on the shared face, edge or corner).
+template <int R, int C>
+__device__ __forceinline__ void sum_cells(const NbrItems &nb, const GhostSrc<R> &gs, int t,
+                                          uint32_t (&acc)[C][R]) {
+  constexpr int NT = 512 / C;
+  int li[C];
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    const int c = t + j * NT;
+    li[j] = (((c >> 6) + 1) * kCore + ((c >> 3) & 7) + 1) * kCore + (c & 7) + 1;
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[j][r] = 0u;
+  }
+  add_items<R, C>(gs, nb.first[13], nb.count[13], li, acc);
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    const int c = t + j * NT, lx = c & 7, ly = (c >> 3) & 7, lz = c >> 6;
+    if (lx != 0 && lx != 7 && ly != 0 && ly != 7 && lz != 0 && lz != 7) continue;
+    const int x0 = lx == 0 ? 0 : 1, x1 = lx == 7 ? 2 : 1;
+    const int y0 = ly == 0 ? 0 : 1, y1 = ly == 7 ? 2 : 1;
+    const int z0 = lz == 0 ? 0 : 1, z1 = lz == 7 ? 2 : 1;
+    for (int dz = z0; dz <= z1; dz++)
+      for (int dy = y0; dy <= y1; dy++)
+        for (int dx = x0; dx <= x1; dx++) {
+          const int n = (dz * 3 + dy) * 3 + dx;
+          if (n == 13 || !nb.count[n]) continue;
+          // core coordinates of this cell in brick n (core origin = brick n - 1)
+          const int cx = lx - (dx - 1) * 8 + 1, cy = ly - (dy - 1) * 8 + 1,
+                    cz = lz - (dz - 1) * 8 + 1;
+          const int l1[1] = {(cz * kCore + cy) * kCore + cx};
+          uint32_t a1[1][R];
+#pragma unroll
+          for (int r = 0; r < R; r++) a1[0][r] = 0u;
+          add_items<R, 1>(gs, nb.first[n], nb.count[n], l1, a1);
+#pragma unroll
+          for (int r = 0; r < R; r++) acc[j][r] += a1[0][r];
+        }
+  }
+}
+
+// One listed brick per block of 512 / C threads, C cells per thread;
+// block-uniform.
+template <int C>
 __device__ __forceinline__ void resolve_mass_entry(const Params &P, const GState &g,
                                                    const BrickWork &bw, uint32_t i) {
+  constexpr int NT = 512 / C;
   __shared__ NbrItems nb;
   const int t = threadIdx.x;
   const uint32_t b = bw.list[i];
-  bool any = load_neighbours(bw, b, nb, t);
   const bool ofl = bw.ofl[b] != 0;
+  bool any = load_neighbours(bw, b, nb, t);
   int bx, by, bz;
   brick_coords(bw, b, bx, by, bz);
-  const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
-  const int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
-  if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
-    uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
-    uint32_t sum = 0;
-    if (any) for_covering_items(nb, lx, ly, lz, [&](size_t gi) { sum += (uint32_t)bw.gm[gi]; });
-    if (ofl) {
-      sum += (uint32_t)bw.om[cell];
-      bw.om[cell] = 0;
+  uint32_t acc[C][1];
+  if (any) {
+    const GhostSrc<1> gs = {{bw.gm}};
+    sum_cells<1, C>(nb, gs, t, acc);
+  } else {
+#pragma unroll
+    for (int j = 0; j < C; j++) acc[j][0] = 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    const int c = t + j * NT;
+    const int x = bx * 8 + (c & 7), y = by * 8 + ((c >> 3) & 7), z = bz * 8 + (c >> 6);
+    if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
+      const uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
+      uint32_t sum = acc[j][0];
+      if (ofl) {
+        sum += (uint32_t)bw.om[cell];
+        bw.om[cell] = 0;
+      }
+      g.m[cell] = (int32_t)sum;
     }
-    g.m[cell] = (int32_t)sum;
   }
   if (t == 0) {
     uint8_t a = bw.act[b];
@@ -387,41 +453,48 @@ __device__ __forceinline__ void resolve_mass_entry(const Params &P, const GState
 
 // Momentum resolve fused with the grid update (WGSL_UPDATE_GRID,
 // shaders.js:443-512): (vx, vy, vz, decoded mass) per cell.  One listed
-// brick per block (512 threads); block-uniform.
+// brick per block of 512 / C threads, C cells per thread; block-uniform.
+template <int C>
 __device__ __forceinline__ void resolve_update_entry(const Params &P, const GState &g,
                                                      const BrickWork &bw, const Walls &W,
                                                      int keep, uint32_t i) {
+  constexpr int NT = 512 / C;
   __shared__ NbrItems nb;
   const int t = threadIdx.x;
   const uint32_t b = bw.list[i];
-  bool any = load_neighbours(bw, b, nb, t);
   const bool ofl = bw.ofl[b] != 0;
+  bool any = load_neighbours(bw, b, nb, t);
   int bx, by, bz;
   brick_coords(bw, b, bx, by, bz);
-  const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
-  const int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
-  if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
-    uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
-    uint32_t sx = 0, sy = 0, sz = 0;
-    if (any)
-      for_covering_items(nb, lx, ly, lz, [&](size_t gi) {
-        sx += (uint32_t)bw.gx[gi];
-        sy += (uint32_t)bw.gy[gi];
-        sz += (uint32_t)bw.gz[gi];
-      });
-    if (ofl) {
-      sx += (uint32_t)bw.ox[cell];
-      sy += (uint32_t)bw.oy[cell];
-      sz += (uint32_t)bw.oz[cell];
-      bw.ox[cell] = 0;
-      bw.oy[cell] = 0;
-      bw.oz[cell] = 0;
-    }
-    g.v[cell] = update_cell(P, W, g.m[cell], (int32_t)sx, (int32_t)sy, (int32_t)sz, x, y, z);
-    if (keep) {
-      g.mx[cell] = (int32_t)sx;
-      g.my[cell] = (int32_t)sy;
-      g.mz[cell] = (int32_t)sz;
+  uint32_t acc[C][3];
+  if (any) {
+    const GhostSrc<3> gs = {{bw.gx, bw.gy, bw.gz}};
+    sum_cells<3, C>(nb, gs, t, acc);
+  } else {
+#pragma unroll
+    for (int j = 0; j < C; j++) acc[j][0] = acc[j][1] = acc[j][2] = 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    const int c = t + j * NT;
+    const int x = bx * 8 + (c & 7), y = by * 8 + ((c >> 3) & 7), z = bz * 8 + (c >> 6);
+    if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
+      const uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
+      uint32_t sx = acc[j][0], sy = acc[j][1], sz = acc[j][2];
+      if (ofl) {
+        sx += (uint32_t)bw.ox[cell];
+        sy += (uint32_t)bw.oy[cell];
+        sz += (uint32_t)bw.oz[cell];
+        bw.ox[cell] = 0;
+        bw.oy[cell] = 0;
+        bw.oz[cell] = 0;
+      }
+      g.v[cell] = update_cell(P, W, g.m[cell], (int32_t)sx, (int32_t)sy, (int32_t)sz, x, y, z);
+      if (keep) {
+        g.mx[cell] = (int32_t)sx;
+        g.my[cell] = (int32_t)sy;
+        g.mz[cell] = (int32_t)sz;
+      }
     }
   }
   if (t == 0) bw.ofl[b] = 0;
@@ -534,8 +607,8 @@ __device__ __forceinline__ void write_items(const BrickWork &bw, uint32_t r, uin
                                             uint32_t item0) {
   const uint32_t ch = 1u << chs;
   const uint32_t pieces = (cnt + ch - 1) >> chs;
-  bw.first_w[r] = cnt ? item0 : 0u;
-  bw.count_w[r] = pieces;
+  bw.first_w[b] = cnt ? item0 : 0u;
+  bw.count_w[b] = pieces;
   bw.occ_w[b] = cnt ? 1 : 0;
   for (uint32_t p = 0; p < pieces; p++) {
     const uint32_t c = cnt - p * ch;
