@@ -504,7 +504,7 @@ __device__ __forceinline__ void resolve_update_entry(const Params &P, const GSta
 // ---------------------------------------------------------------------------
 // G2P over brick work items (WGSL_G2P, shaders.js:589-704).  With src_of set,
 // slot d of the destination state takes the particle at old slot src_of[d]
-// (re-sort; mass, id and density move along) and every particle lies in the
+// (re-sort; id, density and a non-unit mass move along) and every particle lies in the
 // item's brick (EXACT: no particle can leave the window).  Otherwise G2P runs
 // in place over the current items with the drift margin.
 template <int M, bool EXACT>
@@ -552,7 +552,7 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
     in.py = S.at(PY, in.s);
     in.pz = S.at(PZ, in.s);
     if (EXACT) {
-      in.m = S.at(PM, in.s);
+      in.m = P.unit_mass ? 1.0f : S.at(PM, in.s);  // unit mass: the plane is never read
       in.id = S.id[in.s];
       in.dens = S.density[in.s];
     }
@@ -579,7 +579,7 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
     }
     store_g2p(D, d, o);
     if (EXACT) {
-      D.at(PM, d) = cur.m;
+      if (!P.unit_mass) D.at(PM, d) = cur.m;
       D.id[d] = cur.id;
       D.density[d] = cur.dens;
     }
