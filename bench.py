@@ -121,8 +121,9 @@ def batch_bench(args) -> dict:
     interior), every scene re-sorting every step.  Batched: all scenes in one
     mpm_batch (one persistent launch per stage, captured graphs, launch shapes
     from the autotuner).  Sequential: each scene stepped alone (its own
-    captured graphs), one scene after another.  Rates are particle-steps/s
-    over the same number of steps."""
+    captured graphs), one scene after another.  Both forms run the same
+    `batch_steps` steps from the same initial state (a warm-up captures the
+    graphs, then every scene is reset); rates are particle-steps/s."""
     import mlsmpm
     from mlsmpm import autotune, scenes
 
@@ -134,13 +135,20 @@ def batch_bench(args) -> dict:
         s = mlsmpm.Scene(mlsmpm.default_config(*grid, len(pos), sort_period=1, device=0))
         s.reset(pos, vel)
         sc.append(s)
+
+    def reset_all():
+        for s, (_, pos, vel, _) in zip(sc, mix):
+            s.reset(pos, vel)
+
     K, W = args.batch_steps, 17
     b = mlsmpm.Batch()
     for s in sc:
         b.add(s)
     tune = autotune.tune_batch(b)
+    reset_all()
     b.step(W)
     b.sync()
+    reset_all()
     t0 = time.perf_counter()
     b.step(K)
     b.sync()
@@ -149,10 +157,12 @@ def batch_bench(args) -> dict:
     b.step(min(K, 32))
     lat = b.step_times()
     b.close()
+    reset_all()
     for s in sc:
         s.step(W)
     for s in sc:
         s.sync()
+    reset_all()
     t0 = time.perf_counter()
     for s in sc:
         s.step(K)
@@ -187,8 +197,9 @@ def serve_bench(args) -> dict:
     Reports aggregate particle-steps/s (over the loop's busy time and over the
     wall time from first arrival to last completion), p50/p99 batched-step
     latency and p50/p99 request latency; `vs_plain` divides the busy-time rate
-    by the same requests' plain single-scene rate (each scene's per-step time
-    measured alone on a seeded sample of 48 requests, times its step count)."""
+    by the same requests' plain single-scene rate (each scene's time
+    measured alone on a seeded sample of 48 requests: its full step count
+    from the initial state)."""
     import mlsmpm
     from mlsmpm import scenes
 
@@ -218,8 +229,9 @@ def serve_bench(args) -> dict:
     st = srv.stats()
     step_ms = srv.step_times()
     srv.close()
-    # plain form: a seeded sample of the requests, each scene alone, per-step
-    # time over 32 graph-replayed steps
+    # plain form: a seeded sample of the requests, each scene alone running
+    # its request's steps from the initial state (after a warm-up that
+    # captures its graphs and a reset)
     sample = np.random.default_rng(args.seed + 13).choice(nreq, min(nreq, 48), replace=False)
     plain_s, plain_work = 0.0, 0
     for i in sample:
@@ -227,11 +239,11 @@ def serve_bench(args) -> dict:
         with mlsmpm.Scene(mlsmpm.default_config(*grid, len(pos), sort_period=1)) as s:
             s.reset(pos, vel)
             s.step(17)
-            s.sync()
+            s.reset(pos, vel)
             ta = time.perf_counter()
-            s.step(32)
+            s.step(int(steps[i]))
             s.sync()
-            plain_s += (time.perf_counter() - ta) / 32 * int(steps[i])
+            plain_s += time.perf_counter() - ta
             plain_work += len(pos) * int(steps[i])
     plain_rate = plain_work / plain_s
     busy_rate = st["particle_steps"] / st["busy_seconds"]
@@ -278,7 +290,7 @@ def main() -> int:
     ap.add_argument("--no-events", action="store_true",
                     help="skip the event-timed region (no stage times, no roofline)")
     ap.add_argument("--batch-scenes", type=int, default=256)
-    ap.add_argument("--batch-steps", type=int, default=48)
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0)

@@ -190,6 +190,8 @@ struct mpm_batch {
   void *dev = nullptr;              // descriptor tables and prefixes
   size_t dev_bytes = 0;
   BatchTables tab[2]{};
+  std::vector<SceneDev> host_tab;   // uploaded descriptors, phase 0 then phase 1
+  std::vector<uint32_t> host_pre;   // uploaded prefixes
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // by start phase
   uint32_t graph_len = 0;
   bool timing = false;
@@ -240,7 +242,6 @@ int fold_events(mpm_batch *b) {
 // Descriptor tables and unit prefixes of the current members.
 int build_tables(mpm_batch *b) {
   BHIP(hipStreamSynchronize(b->stream));
-  drop_graphs(b);
   b->active.clear();
   for (mpm_scene *s : b->mem)
     if (s->n) b->active.push_back(s);
@@ -261,22 +262,40 @@ int build_tables(mpm_batch *b) {
     ppb[i + 1] = ppb[i] + cdiv32(s->n, 256);
   }
   const size_t tbytes = (size_t)ns * sizeof(SceneDev);
-  const size_t need = 2 * tbytes + pre.size() * 4 + 256;
-  if (need > b->dev_bytes) {
-    if (b->dev) BHIP(hipFree(b->dev));
-    b->dev = nullptr;
-    b->dev_bytes = 0;
-    BHIP(hipMalloc(&b->dev, need));
-    b->dev_bytes = need;
+  std::vector<SceneDev> hall(h0);
+  hall.insert(hall.end(), h1.begin(), h1.end());
+  // A rebuild that reproduces the uploaded tables (members reset to the same
+  // particle counts and buffer parities, parameters set to equal values)
+  // keeps them and the graphs captured over them: descriptors are zeroed
+  // before they are filled, so equal bytes mean equal launches.
+  const bool same = b->dev && pre == b->host_pre && hall.size() == b->host_tab.size() &&
+                    (hall.empty() ||
+                     memcmp(hall.data(), b->host_tab.data(), hall.size() * sizeof(SceneDev)) == 0);
+  if (!same) {
+    drop_graphs(b);
+    const size_t need = 2 * tbytes + pre.size() * 4 + 256;
+    if (need > b->dev_bytes) {
+      if (b->dev) BHIP(hipFree(b->dev));
+      b->dev = nullptr;
+      b->dev_bytes = 0;
+      b->host_tab.clear();
+      b->host_pre.clear();
+      BHIP(hipMalloc(&b->dev, need));
+      b->dev_bytes = need;
+    }
   }
   char *base = (char *)b->dev;
   SceneDev *d0 = (SceneDev *)base, *d1 = (SceneDev *)(base + tbytes);
   uint32_t *dpre = (uint32_t *)(base + 2 * tbytes);
-  if (ns) {
+  if (ns && !same) {
     BHIP(hipMemcpyAsync(d0, h0.data(), tbytes, hipMemcpyHostToDevice, b->stream));
     BHIP(hipMemcpyAsync(d1, h1.data(), tbytes, hipMemcpyHostToDevice, b->stream));
     BHIP(hipMemcpyAsync(dpre, pre.data(), pre.size() * 4, hipMemcpyHostToDevice, b->stream));
     BHIP(hipStreamSynchronize(b->stream));
+  }
+  if (!same) {
+    b->host_tab.swap(hall);
+    b->host_pre = pre;
   }
   for (int ph = 0; ph < 2; ph++) {
     BatchTables &t = b->tab[ph];
