@@ -50,6 +50,10 @@ struct BatchTables {
   uint32_t ns;
   uint32_t tot_items, tot_bricks, tot_bblocks, tot_pblocks;
   uint32_t all_unit;            // every scene has unit particle mass
+  // The step's resolve list over all scenes, (scene, brick) pairs
+  // (capacity tot_bricks), and its length; empty at every step start.
+  uint2 *list;
+  uint32_t *nlist;
 };
 
 // One batched step (sort period 1: every step re-sorts inside G2P) of every

@@ -65,9 +65,12 @@ __global__ __launch_bounds__(256) void k_batch_mom(const SceneDev *__restrict__ 
   }
 }
 
+// Also empties the batch resolve list for the next step (the resolves of
+// this one have run).
 __global__ __launch_bounds__(256) void k_batch_g2p(const SceneDev *__restrict__ sd,
                                                    const uint32_t *__restrict__ pre, uint32_t ns,
-                                                   uint32_t total) {
+                                                   uint32_t total, uint32_t *nlist) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nlist = 0u;
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     const uint32_t si = find_scene(pre, ns, u);
     const SceneDev &d = sd[si];
@@ -76,34 +79,38 @@ __global__ __launch_bounds__(256) void k_batch_g2p(const SceneDev *__restrict__ 
   }
 }
 
+// The bricks of every scene that the resolves must visit, appended as
+// (scene, brick) pairs to one list, so the resolves walk only listed bricks.
 __global__ __launch_bounds__(256) void k_batch_brick_list(const SceneDev *__restrict__ sd,
                                                           const uint32_t *__restrict__ pre,
-                                                          uint32_t ns, uint32_t total) {
+                                                          uint32_t ns, uint32_t total,
+                                                          uint2 *list, uint32_t *nlist) {
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     const uint32_t si = find_scene(pre, ns, u);
-    brick_list_one(sd[si].bw, (u - pre[si]) * 256u + threadIdx.x);
+    const uint32_t b = (u - pre[si]) * 256u + threadIdx.x;
+    if (brick_needed(sd[si].bw, b)) list[atomicAdd(nlist, 1u)] = make_uint2(si, b);
   }
 }
 
-__global__ __launch_bounds__(kResolveThreads) void k_batch_resolve_mass(const SceneDev *__restrict__ sd,
-                                                            const uint32_t *__restrict__ pre,
-                                                            uint32_t ns, uint32_t total) {
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const uint32_t si = find_scene(pre, ns, u);
+__global__ __launch_bounds__(kResolveThreads) void k_batch_resolve_mass(
+    const SceneDev *__restrict__ sd, const uint2 *__restrict__ list, const uint32_t *nlist) {
+  const uint32_t nl = *nlist;
+  for (uint32_t u = blockIdx.x; u < nl; u += gridDim.x) {
+    const uint2 e = list[u];
+    const uint32_t si = __builtin_amdgcn_readfirstlane(e.x), b = __builtin_amdgcn_readfirstlane(e.y);
     const SceneDev &d = sd[si];
-    const uint32_t i = u - pre[si];
-    if (i < *d.bw.nlist) resolve_mass_entry<kResolveCells>(d.P, d.g, d.bw, i);
+    resolve_mass_entry<kResolveCells>(d.P, d.g, d.bw, b);
   }
 }
 
-__global__ __launch_bounds__(kResolveThreads) void k_batch_resolve_update(const SceneDev *__restrict__ sd,
-                                                              const uint32_t *__restrict__ pre,
-                                                              uint32_t ns, uint32_t total) {
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const uint32_t si = find_scene(pre, ns, u);
+__global__ __launch_bounds__(kResolveThreads) void k_batch_resolve_update(
+    const SceneDev *__restrict__ sd, const uint2 *__restrict__ list, const uint32_t *nlist) {
+  const uint32_t nl = *nlist;
+  for (uint32_t u = blockIdx.x; u < nl; u += gridDim.x) {
+    const uint2 e = list[u];
+    const uint32_t si = __builtin_amdgcn_readfirstlane(e.x), b = __builtin_amdgcn_readfirstlane(e.y);
     const SceneDev &d = sd[si];
-    const uint32_t i = u - pre[si];
-    if (i < *d.bw.nlist) resolve_update_entry<kResolveCells>(d.P, d.g, d.bw, d.W, d.keep, i);
+    resolve_update_entry<kResolveCells>(d.P, d.g, d.bw, d.W, d.keep, b);
   }
 }
 
@@ -158,14 +165,14 @@ int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &gr
   k_batch_build_src<<<gp, 256, 0, side>>>(t.sd, t.pre_pblocks, ns, t.tot_pblocks);
   e = hipEventRecord(join, side);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step join");
-  k_batch_brick_list<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks);
-  k_batch_resolve_mass<<<gr, kResolveThreads, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
+  k_batch_brick_list<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks, t.list, t.nlist);
+  k_batch_resolve_mass<<<gr, kResolveThreads, 0, st>>>(t.sd, t.list, t.nlist);
   MPM_BATCH_ITEMS(k_batch_mom);
 #undef MPM_BATCH_ITEMS
-  k_batch_resolve_update<<<gr, kResolveThreads, 0, st>>>(t.sd, t.pre_bricks, ns, t.tot_bricks);
+  k_batch_resolve_update<<<gr, kResolveThreads, 0, st>>>(t.sd, t.list, t.nlist);
   e = hipStreamWaitEvent(st, join, 0);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step wait");
-  k_batch_g2p<<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);
+  k_batch_g2p<<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items, t.nlist);
   return MPM_OK;
 }
 
@@ -240,6 +247,12 @@ int fold_events(mpm_batch *b) {
 }
 
 // Descriptor tables and unit prefixes of the current members.
+// Byte offset of the batch resolve list in the table allocation: the two
+// descriptor tables, the prefixes, then 256 bytes holding the list length.
+static size_t list_off(size_t tbytes, size_t npre) {
+  return ((2 * tbytes + npre * 4 + 255) & ~(size_t)255) + 256;
+}
+
 int build_tables(mpm_batch *b) {
   BHIP(hipStreamSynchronize(b->stream));
   b->active.clear();
@@ -273,7 +286,7 @@ int build_tables(mpm_batch *b) {
                      memcmp(hall.data(), b->host_tab.data(), hall.size() * sizeof(SceneDev)) == 0);
   if (!same) {
     drop_graphs(b);
-    const size_t need = 2 * tbytes + pre.size() * 4 + 256;
+    const size_t need = list_off(tbytes, pre.size()) + (size_t)pb[ns] * sizeof(uint2) + 256;
     if (need > b->dev_bytes) {
       if (b->dev) BHIP(hipFree(b->dev));
       b->dev = nullptr;
@@ -287,10 +300,13 @@ int build_tables(mpm_batch *b) {
   char *base = (char *)b->dev;
   SceneDev *d0 = (SceneDev *)base, *d1 = (SceneDev *)(base + tbytes);
   uint32_t *dpre = (uint32_t *)(base + 2 * tbytes);
+  uint32_t *dnlist = (uint32_t *)(base + list_off(tbytes, pre.size()) - 256);
+  uint2 *dlist = (uint2 *)(base + list_off(tbytes, pre.size()));
   if (ns && !same) {
     BHIP(hipMemcpyAsync(d0, h0.data(), tbytes, hipMemcpyHostToDevice, b->stream));
     BHIP(hipMemcpyAsync(d1, h1.data(), tbytes, hipMemcpyHostToDevice, b->stream));
     BHIP(hipMemcpyAsync(dpre, pre.data(), pre.size() * 4, hipMemcpyHostToDevice, b->stream));
+    BHIP(hipMemsetAsync(dnlist, 0, 4, b->stream));
     BHIP(hipStreamSynchronize(b->stream));
   }
   if (!same) {
@@ -310,6 +326,8 @@ int build_tables(mpm_batch *b) {
     t.tot_bblocks = pbb[ns];
     t.tot_pblocks = ppb[ns];
     t.all_unit = all_unit;
+    t.list = dlist;
+    t.nlist = dnlist;
   }
   b->seen.resize(b->mem.size());
   for (size_t i = 0; i < b->mem.size(); i++) b->seen[i] = b->mem[i]->version;

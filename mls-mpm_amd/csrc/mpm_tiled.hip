@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve_mass(Params P, GSta
                                                                   BrickWork bw) {
   const uint32_t nl = *bw.nlist;
   for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x)
-    resolve_mass_entry<kResolveCells>(P, g, bw, i);
+    resolve_mass_entry<kResolveCells>(P, g, bw, bw.list[i]);
 }
 
 __global__ __launch_bounds__(kResolveThreads) void k_resolve_update(Params P, GState g,
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve_update(Params P, GS
                                                                     int keep) {
   const uint32_t nl = *bw.nlist;
   for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x)
-    resolve_update_entry<kResolveCells>(P, g, bw, W, keep, i);
+    resolve_update_entry<kResolveCells>(P, g, bw, W, keep, bw.list[i]);
 }
 
 // First-step keys: brick and rank of every particle.  Neighbouring slots

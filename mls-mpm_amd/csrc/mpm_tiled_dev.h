@@ -294,8 +294,8 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
 // were non-zero last step: bricks next to an occupied brick, flagged bricks
 // and bricks active last step.  One block (256 threads, two cells each) per
 // listed brick, persistent over the list.
-__device__ __forceinline__ void brick_list_one(const BrickWork &bw, uint32_t b) {
-  if (b >= bw.nbricks) return;
+__device__ __forceinline__ bool brick_needed(const BrickWork &bw, uint32_t b) {
+  if (b >= bw.nbricks) return false;
   int bx, by, bz;
   brick_coords(bw, b, bx, by, bz);
   bool need = (bw.act[b] & 1) || bw.ofl[b];
@@ -305,7 +305,11 @@ __device__ __forceinline__ void brick_list_one(const BrickWork &bw, uint32_t b) 
       continue;
     need = bw.occ[brick_index(bw, x, y, z)] != 0;
   }
-  if (need) bw.list[atomicAdd(bw.nlist, 1u)] = b;
+  return need;
+}
+
+__device__ __forceinline__ void brick_list_one(const BrickWork &bw, uint32_t b) {
+  if (brick_needed(bw, b)) bw.list[atomicAdd(bw.nlist, 1u)] = b;
 }
 
 struct NbrItems {
@@ -346,89 +350,103 @@ struct GhostSrc {
   const int32_t *p[R];
 };
 
-// Adds the ghost values of items [f, f + cn) at core offsets li[0..C-1] into
-// acc.  Four items per round: the item index is clamped into the range and
-// the excess masked to zero, so every load is in bounds and a thread keeps
-// 4 * C * R loads in flight.  Integer sums: the order is free.
-template <int R, int C>
-__device__ __forceinline__ void add_items(const GhostSrc<R> &gs, uint32_t f, uint32_t cn,
-                                          const int (&li)[C], uint32_t (&acc)[C][R]) {
-  for (uint32_t k = 0; k < cn; k += 4) {
+// Ghost sums of every cell of the brick into acc[R][512] (LDS), by the whole
+// block.  The contributions are the brick's own items (all 512 cells) and,
+// for each neighbour brick n, its items' halo cells that lie in this brick
+// (a face of 64 cells, an edge of 8 or a corner).  They are numbered flat
+// (neighbour, item, cell) through a per-neighbour prefix, and every thread
+// takes four entries at a time: decode, issue all their loads, then add
+// into LDS, so a block keeps many independent loads in flight instead of
+// walking the neighbours one after another.  Integer sums: the order is
+// free.  Block-uniform; ends with a barrier.
+template <int R>
+__device__ __forceinline__ void gather_cells(const NbrItems &nb, const GhostSrc<R> &gs,
+                                             int32_t (*acc)[512], int t, int nt) {
+  __shared__ uint32_t pre[28];
+  if (t < 64) {
+    // work of neighbour t: items x cells of its region in this brick
+    uint32_t w = 0;
+    if (t < 27) {
+      const int m = (t % 3 != 1) + ((t / 3) % 3 != 1) + (t / 9 != 1);
+      w = nb.count[t] << (9 - 3 * m);
+    }
+    // inclusive scan over the wavefront, stored exclusive
+    uint32_t x = w;
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, 64);
+      if (t >= off) x += y;
+    }
+    if (t < 28) pre[t] = x - w;
+  }
+  for (int c = t; c < R * 512; c += nt) acc[c >> 9][c & 511] = 0;
+  __syncthreads();
+  const uint32_t total = pre[27];
+  for (uint32_t f0 = t; f0 < total; f0 += 4u * nt) {
+    size_t src[4];
+    int tgt[4];
+    bool ok[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const uint32_t kk = k + u < cn ? k + u : cn - 1u;
-      const uint32_t keep = k + u < cn ? ~0u : 0u;
-      const size_t base = (size_t)(f + kk) * kCoreV;
+      const uint32_t f = f0 + u * nt;
+      ok[u] = f < total;
+      const uint32_t ff = ok[u] ? f : total - 1u;
+      int lo = 0, hi = 27;  // pre[lo] <= ff < pre[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= ff) lo = mid;
+        else hi = mid;
+      }
+      const int n = lo, d[3] = {n % 3, (n / 3) % 3, n / 9};
+      const int m = (d[0] != 1) + (d[1] != 1) + (d[2] != 1), sh = 9 - 3 * m;
+      const uint32_t r = ff - pre[n], k = r >> sh, q = r & ((1u << sh) - 1u);
+      int l[3], cc[3], j = 0;
 #pragma unroll
-      for (int j = 0; j < C; j++)
-#pragma unroll
-        for (int r = 0; r < R; r++) acc[j][r] += (uint32_t)gs.p[r][base + li[j]] & keep;
-    }
-  }
-}
-
-// Ghost sums of the C cells c_j = t + j * (512 / C) of the brick: the
-// brick's own items cover every cell; the items of a neighbour brick reach
-// one cell into it (the cells on the shared face, edge or corner).
-template <int R, int C>
-__device__ __forceinline__ void sum_cells(const NbrItems &nb, const GhostSrc<R> &gs, int t,
-                                          uint32_t (&acc)[C][R]) {
-  constexpr int NT = 512 / C;
-  int li[C];
-#pragma unroll
-  for (int j = 0; j < C; j++) {
-    const int c = t + j * NT;
-    li[j] = (((c >> 6) + 1) * kCore + ((c >> 3) & 7) + 1) * kCore + (c & 7) + 1;
-#pragma unroll
-    for (int r = 0; r < R; r++) acc[j][r] = 0u;
-  }
-  add_items<R, C>(gs, nb.first[13], nb.count[13], li, acc);
-#pragma unroll
-  for (int j = 0; j < C; j++) {
-    const int c = t + j * NT, lx = c & 7, ly = (c >> 3) & 7, lz = c >> 6;
-    if (lx != 0 && lx != 7 && ly != 0 && ly != 7 && lz != 0 && lz != 7) continue;
-    const int x0 = lx == 0 ? 0 : 1, x1 = lx == 7 ? 2 : 1;
-    const int y0 = ly == 0 ? 0 : 1, y1 = ly == 7 ? 2 : 1;
-    const int z0 = lz == 0 ? 0 : 1, z1 = lz == 7 ? 2 : 1;
-    for (int dz = z0; dz <= z1; dz++)
-      for (int dy = y0; dy <= y1; dy++)
-        for (int dx = x0; dx <= x1; dx++) {
-          const int n = (dz * 3 + dy) * 3 + dx;
-          if (n == 13 || !nb.count[n]) continue;
-          // core coordinates of this cell in brick n (core origin = brick n - 1)
-          const int cx = lx - (dx - 1) * 8 + 1, cy = ly - (dy - 1) * 8 + 1,
-                    cz = lz - (dz - 1) * 8 + 1;
-          const int l1[1] = {(cz * kCore + cy) * kCore + cx};
-          uint32_t a1[1][R];
-#pragma unroll
-          for (int r = 0; r < R; r++) a1[0][r] = 0u;
-          add_items<R, 1>(gs, nb.first[n], nb.count[n], l1, a1);
-#pragma unroll
-          for (int r = 0; r < R; r++) acc[j][r] += a1[0][r];
+      for (int a = 0; a < 3; a++) {
+        // free axis: coordinate from q; neighbour below: our low face, its
+        // core's high halo (9); neighbour above: our high face, its core's 0
+        if (d[a] == 1) {
+          l[a] = (int)(q >> (3 * j)) & 7;
+          cc[a] = l[a] + 1;
+          j++;
+        } else {
+          l[a] = d[a] == 0 ? 0 : 7;
+          cc[a] = d[a] == 0 ? 9 : 0;
         }
+      }
+      tgt[u] = (l[2] * 8 + l[1]) * 8 + l[0];
+      src[u] = (size_t)(nb.first[n] + k) * kCoreV + (size_t)((cc[2] * kCore + cc[1]) * kCore + cc[0]);
+    }
+    int32_t v[4][R];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int r = 0; r < R; r++) v[u][r] = gs.p[r][src[u]];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (ok[u])
+#pragma unroll
+        for (int r = 0; r < R; r++) atomicAdd(&acc[r][tgt[u]], v[u][r]);
   }
+  __syncthreads();
 }
 
-// One listed brick per block of 512 / C threads, C cells per thread;
+// Listed brick b, one block of 512 / C threads, C cells per thread;
 // block-uniform.
 template <int C>
 __device__ __forceinline__ void resolve_mass_entry(const Params &P, const GState &g,
-                                                   const BrickWork &bw, uint32_t i) {
+                                                   const BrickWork &bw, uint32_t b) {
   constexpr int NT = 512 / C;
   __shared__ NbrItems nb;
   const int t = threadIdx.x;
-  const uint32_t b = bw.list[i];
   const bool ofl = bw.ofl[b] != 0;
   bool any = load_neighbours(bw, b, nb, t);
   int bx, by, bz;
   brick_coords(bw, b, bx, by, bz);
-  uint32_t acc[C][1];
+  __shared__ int32_t acc[1][512];
   if (any) {
     const GhostSrc<1> gs = {{bw.gm}};
-    sum_cells<1, C>(nb, gs, t, acc);
-  } else {
-#pragma unroll
-    for (int j = 0; j < C; j++) acc[j][0] = 0u;
+    gather_cells<1>(nb, gs, acc, t, NT);
   }
 #pragma unroll
   for (int j = 0; j < C; j++) {
@@ -436,7 +454,7 @@ __device__ __forceinline__ void resolve_mass_entry(const Params &P, const GState
     const int x = bx * 8 + (c & 7), y = by * 8 + ((c >> 3) & 7), z = bz * 8 + (c >> 6);
     if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
       const uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
-      uint32_t sum = acc[j][0];
+      uint32_t sum = any ? (uint32_t)acc[0][c] : 0u;
       if (ofl) {
         sum += (uint32_t)bw.om[cell];
         bw.om[cell] = 0;
@@ -452,27 +470,23 @@ __device__ __forceinline__ void resolve_mass_entry(const Params &P, const GState
 }
 
 // Momentum resolve fused with the grid update (WGSL_UPDATE_GRID,
-// shaders.js:443-512): (vx, vy, vz, decoded mass) per cell.  One listed
-// brick per block of 512 / C threads, C cells per thread; block-uniform.
+// shaders.js:443-512): (vx, vy, vz, decoded mass) per cell.  Listed brick
+// b, one block of 512 / C threads, C cells per thread; block-uniform.
 template <int C>
 __device__ __forceinline__ void resolve_update_entry(const Params &P, const GState &g,
                                                      const BrickWork &bw, const Walls &W,
-                                                     int keep, uint32_t i) {
+                                                     int keep, uint32_t b) {
   constexpr int NT = 512 / C;
   __shared__ NbrItems nb;
   const int t = threadIdx.x;
-  const uint32_t b = bw.list[i];
   const bool ofl = bw.ofl[b] != 0;
   bool any = load_neighbours(bw, b, nb, t);
   int bx, by, bz;
   brick_coords(bw, b, bx, by, bz);
-  uint32_t acc[C][3];
+  __shared__ int32_t acc[3][512];
   if (any) {
     const GhostSrc<3> gs = {{bw.gx, bw.gy, bw.gz}};
-    sum_cells<3, C>(nb, gs, t, acc);
-  } else {
-#pragma unroll
-    for (int j = 0; j < C; j++) acc[j][0] = acc[j][1] = acc[j][2] = 0u;
+    gather_cells<3>(nb, gs, acc, t, NT);
   }
 #pragma unroll
   for (int j = 0; j < C; j++) {
@@ -480,7 +494,8 @@ __device__ __forceinline__ void resolve_update_entry(const Params &P, const GSta
     const int x = bx * 8 + (c & 7), y = by * 8 + ((c >> 3) & 7), z = bz * 8 + (c >> 6);
     if (x < (int)P.gx && y < (int)P.gy && z < (int)P.gz) {
       const uint32_t cell = ((uint32_t)z * P.gy + (uint32_t)y) * P.gx + (uint32_t)x;
-      uint32_t sx = acc[j][0], sy = acc[j][1], sz = acc[j][2];
+      uint32_t sx = any ? (uint32_t)acc[0][c] : 0u, sy = any ? (uint32_t)acc[1][c] : 0u,
+               sz = any ? (uint32_t)acc[2][c] : 0u;
       if (ofl) {
         sx += (uint32_t)bw.ox[cell];
         sy += (uint32_t)bw.oy[cell];
