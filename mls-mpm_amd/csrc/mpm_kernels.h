@@ -13,6 +13,7 @@ struct DiffuseBufs {
   uint32_t *genbits;   // spawner bitmask by reset-order id, ceil(cap/32) words
   uint32_t *gensrc;    // slot of each spawner, by id (cap words)
   uint32_t *killbits;  // killed-slot bitmask, ceil(dmax/32) words
+  uint32_t *alivebits; // alive flag of every slot (a copy of the records'), ceil(dmax/32) words
   uint32_t *prefix;    // scan output, max(ceil(cap/32), ceil(dmax/32)) words
   uint32_t *scratch;   // scan tile sums
   uint32_t *total;     // scan total
@@ -39,6 +40,8 @@ void launch_g2p_global(const Params &P, const PState &S, const GState &g, const 
                        hipStream_t st);
 void launch_g2p_relayout(const Params &P, const PState &S, const PState &D, const GState &g,
                          const Walls &W, const uint32_t *src_of, hipStream_t st);
+// Rebuilds the alive bitmask from the pool records (after a host write).
+void launch_alive_bits(const DiffuseBufs &D, uint32_t dmax, hipStream_t st);
 void launch_diffuse(const Params &P, const PState &S, const GState &g, const Walls &W,
                     const DiffuseBufs &D, hipStream_t st);
 uint32_t scan_scratch_words(uint32_t n);

@@ -208,7 +208,8 @@ __global__ __launch_bounds__(256) void k_gen_apply(Params P, PState S, GState g,
                                                    const uint32_t *seedp, uint32_t *bits,
                                                    const uint32_t *prefix, const uint32_t *src,
                                                    const uint32_t *free_list, const int32_t *topp,
-                                                   DRec *pool, uint32_t nwords) {
+                                                   DRec *pool, uint32_t *alivebits,
+                                                   uint32_t nwords) {
   uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwords) return;
   uint32_t word = bits[w];
@@ -247,6 +248,7 @@ __global__ __launch_bounds__(256) void k_gen_apply(Params P, PState S, GState g,
     rec.lifetime = lifetime;
     rec.pad = 0.0f;
     pool[slot] = rec;
+    atomicOr(&alivebits[slot >> 5], 1u << (slot & 31));
   }
 }
 
@@ -295,11 +297,13 @@ __device__ __forceinline__ void reflect_axis(float &p, float &v, float lo, float
   v = nv;
 }
 
+// The alive bitmask mirrors the records' alive flags, so dead slots are
+// skipped without touching the pool (32 slots per word read).
 __global__ __launch_bounds__(256) void k_advect(Params P, GState g, DRec *pool, uint32_t *killbits,
-                                                Walls W) {
+                                                uint32_t *alivebits, Walls W) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.dmax) return;
-  if (pool[i].alive == 0u) return;
+  if (!((alivebits[i >> 5] >> (i & 31)) & 1u)) return;
   DRec p = pool[i];
   const float dt = P.dt, gr = P.gravity;
   F3 pos3 = f3(p.pos.x, p.pos.y, p.pos.z);
@@ -342,6 +346,7 @@ __global__ __launch_bounds__(256) void k_advect(Params P, GState g, DRec *pool, 
   }
   if (kill) {  // kill_particle (shaders.js:914-918): payload kept, alive = 0
     pool[i].alive = 0u;
+    atomicAnd(&alivebits[i >> 5], ~(1u << (i & 31)));
     atomicOr(&killbits[i >> 5], 1u << (i & 31));
     return;
   }
@@ -351,6 +356,17 @@ __global__ __launch_bounds__(256) void k_advect(Params P, GState g, DRec *pool, 
   p.pos.w = 0.0f;
   p.vel.w = 0.0f;
   pool[i] = p;
+}
+
+// Alive bitmask from the records (after the host wrote the pool).
+__global__ __launch_bounds__(256) void k_alive_bits(const DRec *pool, uint32_t *alivebits,
+                                                    uint32_t dmax) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= (dmax + 31u) / 32u) return;
+  uint32_t bits = 0;
+  for (uint32_t b = 0; b < 32u && w * 32u + b < dmax; b++)
+    if (pool[w * 32u + b].alive != 0u) bits |= 1u << b;
+  alivebits[w] = bits;
 }
 
 __global__ __launch_bounds__(256) void k_kill_push(uint32_t *killbits, const uint32_t *prefix,
@@ -578,6 +594,10 @@ void launch_scan(const uint32_t *in, uint32_t n, bool popc, uint32_t *out, uint3
   else k_scan_down<false><<<nb, kScanThreads, 0, st>>>(in, n, scratch, out, clear);
 }
 
+void launch_alive_bits(const DiffuseBufs &D, uint32_t dmax, hipStream_t st) {
+  if (dmax) k_alive_bits<<<blocks((dmax + 31u) / 32u, 256), 256, 0, st>>>(D.pool, D.alivebits, dmax);
+}
+
 void launch_diffuse(const Params &P, const PState &S, const GState &g, const Walls &W,
                     const DiffuseBufs &D, hipStream_t st) {
   if (!P.dmax) return;
@@ -586,12 +606,12 @@ void launch_diffuse(const Params &P, const PState &S, const GState &g, const Wal
     k_gen_decide<<<blocks(P.n, 256), 256, 0, st>>>(P, S, g, D.seed, D.genbits, D.gensrc);
     launch_scan(D.genbits, nwg, true, D.prefix, D.scratch, D.total, st);
     k_gen_apply<<<blocks(nwg, 256), 256, 0, st>>>(P, S, g, D.seed, D.genbits, D.prefix, D.gensrc,
-                                                  D.free_list, D.top, D.pool, nwg);
+                                                  D.free_list, D.top, D.pool, D.alivebits, nwg);
   } else {
     k_zero_total<<<1, 1, 0, st>>>(D.total);
   }
   k_gen_finish<<<1, 1, 0, st>>>(D.top, D.total, D.seed);
-  k_advect<<<blocks(P.dmax, 256), 256, 0, st>>>(P, g, D.pool, D.killbits, W);
+  k_advect<<<blocks(P.dmax, 256), 256, 0, st>>>(P, g, D.pool, D.killbits, D.alivebits, W);
   uint32_t nwk = blocks(P.dmax, 32);
   launch_scan(D.killbits, nwk, true, D.prefix, D.scratch, D.total, st);
   k_kill_push<<<blocks(nwk, 256), 256, 0, st>>>(D.killbits, D.prefix, D.free_list, D.top, nwk,

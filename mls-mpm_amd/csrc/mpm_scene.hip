@@ -87,7 +87,7 @@ size_t scene_footprint(const mpm_config &c) {
   b += nit * (16 + 4 * 4000);            // work items and ghost cores
   b += nb * 8 + 65536;                   // flags, lists, scan scratch
   b += cap * 72;                         // read-back staging
-  if (D) b += D * (48 + 4) + cap * 4 + (cap / 32 + D / 32 + 2) * 8 + 4096;
+  if (D) b += D * (48 + 4) + cap * 4 + (cap / 32 + 2 * (D / 32) + 3) * 8 + 4096;
   return b;
 }
 
@@ -197,6 +197,7 @@ static int reset_diffuse(mpm_scene *s) {
   std::vector<uint32_t> fl(D);
   for (uint32_t i = 0; i < D; i++) fl[i] = i;
   HIPCHK(hipMemsetAsync(s->d.pool, 0, (size_t)D * sizeof(DRec), s->stream));
+  HIPCHK(hipMemsetAsync(s->d.alivebits, 0, (size_t)((D + 31) / 32) * 4, s->stream));
   HIPCHK(hipMemcpyAsync(s->d.free_list, fl.data(), (size_t)D * 4, hipMemcpyHostToDevice,
                         s->stream));
   int32_t top = (int32_t)D;
@@ -341,6 +342,7 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     DALLOC(s->d.genbits, (size_t)nwg * 4);
     DALLOC(s->d.gensrc, cap * 4);
     DALLOC(s->d.killbits, (size_t)nwk * 4);
+    DALLOC(s->d.alivebits, (size_t)nwk * 4);
     DALLOC(s->d.prefix, (size_t)nw * 4);
     DALLOC(s->d.scratch, (size_t)scan_scratch_words(nw) * 4);
     DALLOC(s->d.total, 4);
@@ -837,8 +839,11 @@ int scene_write_diffuse(mpm_scene *s, const void *rec48, const uint32_t *free_li
         return failf(MPM_ERR_INVALID, "free_list[%u] = %u is not a slot index", i, free_list[i]);
   HIPCHK(hipSetDevice(s->cfg.device));
   HIPCHK(hipStreamSynchronize(s->stream));
-  if (rec48)
+  if (rec48) {
     HIPCHK(hipMemcpyAsync(s->d.pool, rec48, (size_t)D * 48, hipMemcpyHostToDevice, s->stream));
+    launch_alive_bits(s->d, D, s->stream);
+    HIPCHK(hipGetLastError());
+  }
   if (free_list)
     HIPCHK(hipMemcpyAsync(s->d.free_list, free_list, (size_t)D * 4, hipMemcpyHostToDevice,
                           s->stream));

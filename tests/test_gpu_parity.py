@@ -286,6 +286,39 @@ def test_reset_keeps_graphs_and_bits(period):
     f2.close()
 
 
+def test_state_round_trip_with_diffuse_pool():
+    """A scene rebuilt from another's read-back state (particles in reset
+    order, the diffuse records, free list and top, the generator seed)
+    continues bit-identically: mpm_write_diffuse restores which slots are
+    alive, and the step depends on no other carried state."""
+    grid = (40, 32, 36)
+    pos = scenes.lattice_block((2, 2, 2), (22, 22, 20), 0.1, np.random.default_rng(21))
+    D, seed0, n0 = 20000, 5, 30
+    kw = dict(flags=KEEP, sort_period=1, tile_ppt=4, seed=seed0)
+    a, _ = make_pair(grid, pos, D=D, **kw)
+    a.step(n0)
+    rec, fl, top = a.read_diffuse()
+    assert int(np.count_nonzero(rec["alive"])) > 0 and top < D
+    p = a.read_particles()
+    b, _ = make_pair(grid, p["pos4"][:, :3], vel=p["vel4"][:, :3], D=D, **kw)
+    b.reset(p["pos4"][:, :3], p["vel4"][:, :3], p["C9"])
+    b.set_seed(seed0 + n0)
+    b.write_diffuse(rec, fl, top)
+    a.step(25)
+    b.step(25)
+    pa, pb = a.read_particles(), b.read_particles()
+    for k in pa:
+        np.testing.assert_array_equal(pa[k], pb[k], err_msg=f"round trip {k}")
+    ra, fa, ta = a.read_diffuse()
+    rb, fb, tb = b.read_diffuse()
+    assert ta == tb
+    np.testing.assert_array_equal(fa, fb)
+    for f in ("pos", "vel", "ptype", "alive", "lifetime"):
+        np.testing.assert_array_equal(ra[f], rb[f], err_msg=f"round trip diffuse {f}")
+    a.close()
+    b.close()
+
+
 def wall_block(count):
     """WallParams (shaders.js:123-133): header (count) + 4 walls x 20 floats:
     centre, cos yaw, local min, sin yaw, local max, cos pitch, linear
