@@ -159,6 +159,44 @@ int mpm_set_seed(mpm_scene *s, uint32_t seed);
  * (shaders.js:123-133).  NULL clears all walls. */
 int mpm_set_walls(mpm_scene *s, const float *wallp84);
 
+/* ---- Interaction ---------------------------------------------------------
+ * The hand push of FluidGPU.updateHand (fluid-gpu.js:479-527; HandParams,
+ * shaders.js:394-402): `vel` is added to the velocity of every grid cell
+ * within `radius` of the line through `pos` along the eye-to-pos ray, scaled
+ * by (1 - d/radius)^2 * strength, between the grid update and G2P of every
+ * step (WGSL_APPLY_HAND, shaders.js:410-432) until set again with active = 0.
+ * 48 bytes. */
+typedef struct mpm_hand {
+    float pos[3];
+    float radius;
+    float vel[3];
+    uint32_t active;
+    float strength;
+    float eye[3];
+} mpm_hand;
+int mpm_set_hand(mpm_scene *s, const mpm_hand *hand);
+
+/* Replaces FluidGPU.deleteNear + pollDelete (fluid-gpu.js:549-586;
+ * WGSL_COMPACT, shaders.js:521-547): removes the particles within `radius` of
+ * the line through origin along dir.  The survivors keep their reset order
+ * (their ids are renumbered 0..remaining-1) and carry their whole state;
+ * synchronous. */
+int mpm_delete_near(mpm_scene *s, const float origin[3], const float dir[3], float radius,
+                    uint32_t *remaining);
+
+/* Replaces FluidGPU.raycastFluid + pollRaycast (fluid-gpu.js:594-638;
+ * WGSL_RAYCAST, shaders.js:557-577): the smallest t > 0 of dot(p - origin,
+ * dir) over particles within `radius` of the ray axis; *hit = 0 when none.
+ * Synchronous. */
+int mpm_raycast(mpm_scene *s, const float origin[3], const float dir[3], float radius,
+                float *t, int *hit);
+
+/* Replaces the upload of FluidGPU.spawnParticles (fluid-gpu.js:145-180): n
+ * particles (positions inside the hard bounds; vel3 may be NULL) appended
+ * after the current ones with unit mass and C = 0, ids continuing the reset
+ * order.  MPM_ERR_CAPACITY past the capacity. */
+int mpm_append_particles(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3);
+
 /* The scene's hipStream_t, for callers that order their own work after it. */
 int mpm_get_stream(mpm_scene *s, void **hip_stream);
 

@@ -462,6 +462,8 @@ int mpm_batch_add(mpm_batch *b, mpm_scene *s) {
                  "batched scenes re-sort every step on the LDS-staged path (sort_period 1)");
   if (c.diffuse_capacity)
     return bfail(MPM_ERR_UNSUPPORTED, "batched scenes carry no diffuse pool");
+  if (s->hand_on)
+    return bfail(MPM_ERR_UNSUPPORTED, "batched scenes apply no hand push (set it inactive first)");
   if (!b->mem.empty() && s->tc.margin != b->margin)
     return bfail(MPM_ERR_INVALID, "tile_margin %u differs from the batch's %u", s->tc.margin,
                  b->margin);

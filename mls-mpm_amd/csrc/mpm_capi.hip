@@ -215,6 +215,28 @@ int mpm_set_walls(mpm_scene *s, const float *wallp) {
   return scene_set_walls(s, wallp);
 }
 
+int mpm_set_hand(mpm_scene *s, const mpm_hand *hand) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_set_hand: scene is NULL");
+  return scene_set_hand(s, hand);
+}
+
+int mpm_delete_near(mpm_scene *s, const float origin[3], const float dir[3], float radius,
+                    uint32_t *remaining) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_delete_near: scene is NULL");
+  return scene_delete_near(s, origin, dir, radius, remaining);
+}
+
+int mpm_raycast(mpm_scene *s, const float origin[3], const float dir[3], float radius, float *t,
+                int *hit) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_raycast: scene is NULL");
+  return scene_raycast(s, origin, dir, radius, t, hit);
+}
+
+int mpm_append_particles(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_append_particles: scene is NULL");
+  return scene_append(s, n, pos3, vel3);
+}
+
 int mpm_get_stream(mpm_scene *s, void **st) {
   if (!s || !st) return fail(MPM_ERR_INVALID, "mpm_get_stream: NULL argument");
   *st = (void *)s->stream;

@@ -19,6 +19,19 @@ struct DiffuseBufs {
   uint32_t *total;     // scan total
 };
 
+// RayParams (shaders.js:78): origin, cylinder radius, direction.
+struct Ray {
+  float ox, oy, oz, radius, dx, dy, dz, pad;
+};
+
+// The hand push on the device (HandParams, shaders.js:394-402): the box of
+// cells to visit (bmin > bmax: none) was computed on the host.
+struct HandDev {
+  float px, py, pz, radius, vx, vy, vz, strength, ex, ey, ez;
+  uint32_t on;
+  int32_t bmin[3], bmax[3];
+};
+
 // Sort buffers (mpm_tiled.hip).
 struct SortBufs {
   uint32_t *keys;       // per particle: Morton rank of its new brick
@@ -51,7 +64,21 @@ void launch_export_particles(const PState &S, uint32_t n, float4 *pos4, float4 *
                              float *dens, hipStream_t st);
 void launch_export_grid(const GState &g, uint32_t G, int32_t *m, int32_t *mv3, float *mf,
                         float *v3, hipStream_t st);
+// Slots first .. first+n-1 (ids alike) from host-layout arrays.
 void launch_import_particles(const PState &S, uint32_t n, const float *pos3, const float *vel3,
-                             const float *C9, const float *mass, hipStream_t st);
+                             const float *C9, const float *mass, hipStream_t st,
+                             uint32_t first = 0);
+
+// Interaction (mpm_interact.hip).  Hand: between the grid update and G2P;
+// act (brick-sparse update) receives the touched bricks.
+void launch_apply_hand(const Params &P, const GState &g, uint8_t *act, uint32_t nbx,
+                       uint32_t nby, const HandDev *h, hipStream_t st);
+// Survivors of the cylinder r, from S into D in reset order (slot = new id);
+// *total receives their number.  bits/prefix: nwords = ceil(n/32) words.
+void launch_delete_mark_move(const PState &S, const PState &D, uint32_t n, const Ray &r,
+                             uint32_t *bits, uint32_t *prefix, uint32_t *scratch,
+                             uint32_t *total, uint32_t nwords, hipStream_t st);
+// *hit = bits of the smallest t > 0 (0xffffffff: none).
+void launch_raycast(const PState &S, uint32_t n, const Ray &r, uint32_t *hit, hipStream_t st);
 
 }  // namespace mpm

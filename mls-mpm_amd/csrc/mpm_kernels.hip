@@ -532,20 +532,21 @@ __global__ __launch_bounds__(256) void k_export_grid(GState g, uint32_t G, int32
 // Scatter host-order particles into slots (identity order) at reset.
 __global__ __launch_bounds__(256) void k_import_particles(PState S, uint32_t n, const float *pos3,
                                                           const float *vel3, const float *C9,
-                                                          const float *mass) {
+                                                          const float *mass, uint32_t first) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  S.at(PX, p) = pos3[3 * (size_t)p];
-  S.at(PY, p) = pos3[3 * (size_t)p + 1];
-  S.at(PZ, p) = pos3[3 * (size_t)p + 2];
-  S.at(VX, p) = vel3 ? vel3[3 * (size_t)p] : 0.0f;
-  S.at(VY, p) = vel3 ? vel3[3 * (size_t)p + 1] : 0.0f;
-  S.at(VZ, p) = vel3 ? vel3[3 * (size_t)p + 2] : 0.0f;
+  const uint32_t q = first + p;
+  S.at(PX, q) = pos3[3 * (size_t)p];
+  S.at(PY, q) = pos3[3 * (size_t)p + 1];
+  S.at(PZ, q) = pos3[3 * (size_t)p + 2];
+  S.at(VX, q) = vel3 ? vel3[3 * (size_t)p] : 0.0f;
+  S.at(VY, q) = vel3 ? vel3[3 * (size_t)p + 1] : 0.0f;
+  S.at(VZ, q) = vel3 ? vel3[3 * (size_t)p + 2] : 0.0f;
 #pragma unroll
-  for (int k = 0; k < 9; k++) S.at(C00 + k, p) = C9 ? C9[(size_t)p * 9 + k] : 0.0f;
-  S.at(PM, p) = mass ? mass[p] : 1.0f;
-  S.density[p] = 0.0f;
-  S.id[p] = p;
+  for (int k = 0; k < 9; k++) S.at(C00 + k, q) = C9 ? C9[(size_t)p * 9 + k] : 0.0f;
+  S.at(PM, q) = mass ? mass[p] : 1.0f;
+  S.density[q] = 0.0f;
+  S.id[q] = q;
 }
 
 // ---------------------------------------------------------------------------
@@ -628,8 +629,8 @@ void launch_export_grid(const GState &g, uint32_t G, int32_t *m, int32_t *mv3, f
   if (G) k_export_grid<<<blocks(G, 256), 256, 0, st>>>(g, G, m, mv3, mf, v3);
 }
 void launch_import_particles(const PState &S, uint32_t n, const float *pos3, const float *vel3,
-                             const float *C9, const float *mass, hipStream_t st) {
-  if (n) k_import_particles<<<blocks(n, 256), 256, 0, st>>>(S, n, pos3, vel3, C9, mass);
+                             const float *C9, const float *mass, hipStream_t st, uint32_t first) {
+  if (n) k_import_particles<<<blocks(n, 256), 256, 0, st>>>(S, n, pos3, vel3, C9, mass, first);
 }
 
 }  // namespace mpm

@@ -72,6 +72,13 @@ struct mpm_scene {
   uint64_t graph_clock = 0;
   int graph_next = 0;            // timed: event half the next replay uses
   int graph_pending = -1;        // timed: event half not yet folded
+  // Interaction, allocated at first use: the hand push on the device (the
+  // step applies it once it was switched on), and the scratch of deleteNear
+  // and raycastFluid.
+  mpm::HandDev *hand_dev = nullptr;
+  bool hand_on = false;
+  uint32_t *ibits = nullptr, *iprefix = nullptr, *iscratch = nullptr, *itotal = nullptr,
+           *ihit = nullptr;
   // Batch membership: a member's work runs on the batch's stream.
   mpm_batch *batch = nullptr;
   hipStream_t own_stream = nullptr;
@@ -98,6 +105,12 @@ int scene_read_diffuse(mpm_scene *s, void *rec48, uint32_t *free_list, int32_t *
 int scene_write_diffuse(mpm_scene *s, const void *rec48, const uint32_t *free_list, int32_t top);
 int scene_set_seed(mpm_scene *s, uint32_t seed);
 int scene_set_walls(mpm_scene *s, const float *wallp);
+int scene_set_hand(mpm_scene *s, const mpm_hand *hand);
+int scene_delete_near(mpm_scene *s, const float *origin, const float *dir, float radius,
+                      uint32_t *remaining);
+int scene_raycast(mpm_scene *s, const float *origin, const float *dir, float radius, float *t,
+                  int *hit);
+int scene_append(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3);
 int scene_enable_timing(mpm_scene *s, bool on);
 int scene_stage_times(mpm_scene *s, float *ms8);
 int scene_tile_stats(mpm_scene *s, uint64_t out[4]);

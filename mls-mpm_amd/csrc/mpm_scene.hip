@@ -537,8 +537,11 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   if (rc) return rc;
   const uint32_t period = s->cfg.sort_period;
   // The LDS-staged scatter works on brick-sorted particles; without a sort
-  // period the direct per-particle form runs.
-  const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS) && period != 0;
+  // period the direct per-particle form runs, and so does a step without
+  // particles (its dense update leaves the whole grid as an empty step does,
+  // whatever the brick flags say).
+  const bool lds = !(s->cfg.flags & MPM_FLAG_NO_LDS) && period != 0;
+  const bool tiled = lds && s->n;
   // Layout policy: the first step after a reset sorts the caller-order
   // particles with a full sort; afterwards every `period` steps the P2G mass
   // pass records new bricks and G2P writes the new state in sorted order.
@@ -559,7 +562,9 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   }
   const BrickWork bw = bw_of(s, s->cur);
   if ((rc = mark(1))) return rc;
-  if (keep && !tiled) launch_clear_grid(s->g, s->G, st);
+  // the dense update expects zero integer grids; the brick form leaves them
+  // zero only with KEEP_GRID off and outside the bricks it resolved
+  if ((keep || lds) && !tiled) launch_clear_grid(s->g, s->G, st);
   KeyOut ko{};
   if (relayout) ko = key_out(s->sort, bw);
   if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, bw, ko, st);
@@ -580,6 +585,8 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   if ((rc = mark(3))) return rc;
   if (tiled) launch_resolve_update(P, s->g, s->tc, bw, s->walls, keep, st);
   else launch_update_dense(P, s->g, s->walls, !keep, st);
+  if (s->hand_on)
+    launch_apply_hand(P, s->g, lds ? s->bw.act : nullptr, s->bw.nbx, s->bw.nby, s->hand_dev, st);
   if ((rc = mark(4))) return rc;
   if (relayout) HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
   if ((rc = mark(5))) return rc;
@@ -858,6 +865,226 @@ int scene_set_seed(mpm_scene *s, uint32_t seed) {
   HIPCHK(hipSetDevice(s->cfg.device));
   HIPCHK(hipMemcpyAsync(s->d.seed, &seed, 4, hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Interaction (fluid-gpu.js:145-180, 479-638).
+
+// One allocation for the hand parameters and the delete / raycast scratch.
+static int ensure_interact(mpm_scene *s) {
+  if (s->hand_dev) return MPM_OK;
+  const size_t cap = s->cfg.capacity ? s->cfg.capacity : 1;
+  const size_t nw = (cap + 31) / 32, sw = scan_scratch_words((uint32_t)nw);
+  const size_t bytes = 256 + 256 + (2 * nw + sw) * 4;
+  char *base = nullptr;
+  DALLOC(base, bytes);
+  s->hand_dev = (HandDev *)base;
+  s->itotal = (uint32_t *)(base + 256);
+  s->ihit = s->itotal + 1;
+  s->ibits = (uint32_t *)(base + 512);
+  s->iprefix = s->ibits + nw;
+  s->iscratch = s->iprefix + nw;
+  HandDev off{};
+  off.bmax[0] = off.bmax[1] = off.bmax[2] = -1;
+  HIPCHK(hipMemcpy(s->hand_dev, &off, sizeof off, hipMemcpyHostToDevice));
+  return MPM_OK;
+}
+
+// The box of grid cells the push can reach: the line through the hand along
+// the eye ray clipped to the grid grown by the radius (lineBoxIntersect,
+// math.js:149-164), its ends grown by the radius again
+// (FluidGPU.updateHand, fluid-gpu.js:494-523).  In double, like the JS.
+static void hand_box(const mpm_config &c, const mpm_hand &h, HandDev &d) {
+  for (int a = 0; a < 3; a++) {
+    d.bmin[a] = 0;
+    d.bmax[a] = -1;
+  }
+  const double pos[3] = {h.pos[0], h.pos[1], h.pos[2]};
+  const double g[3] = {(double)c.grid_x, (double)c.grid_y, (double)c.grid_z};
+  double dir[3] = {pos[0] - h.eye[0], pos[1] - h.eye[1], pos[2] - h.eye[2]};
+  double dl = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  if (dl == 0.0) dl = 1.0;
+  for (double &v : dir) v /= dl;
+  const double R = h.radius;
+  double tmin = -INFINITY, tmax = INFINITY;
+  for (int a = 0; a < 3; a++) {
+    const double lo = -R, hi = g[a] + R;
+    if (std::fabs(dir[a]) < 1e-9) {
+      if (pos[a] < lo || pos[a] > hi) return;
+      continue;
+    }
+    double t1 = (lo - pos[a]) / dir[a], t2 = (hi - pos[a]) / dir[a];
+    if (t1 > t2) std::swap(t1, t2);
+    tmin = std::max(tmin, t1);
+    tmax = std::min(tmax, t2);
+    if (tmin > tmax) return;
+  }
+  for (int a = 0; a < 3; a++) {
+    const double p0 = pos[a] + tmin * dir[a], p1 = pos[a] + tmax * dir[a];
+    d.bmin[a] = (int32_t)std::max(0.0, std::floor(std::min(p0, p1) - R));
+    d.bmax[a] = (int32_t)std::min(g[a] - 1.0, std::ceil(std::max(p0, p1) + R));
+  }
+}
+
+int scene_set_hand(mpm_scene *s, const mpm_hand *hand) {
+  const bool active = hand && hand->active;
+  if (active) {
+    const float v[] = {hand->pos[0], hand->pos[1], hand->pos[2], hand->radius, hand->vel[0],
+                       hand->vel[1], hand->vel[2], hand->strength, hand->eye[0], hand->eye[1],
+                       hand->eye[2]};
+    for (float x : v)
+      if (!std::isfinite(x)) return failf(MPM_ERR_INVALID, "mpm_set_hand: values must be finite");
+    if (s->batch)
+      return failf(MPM_ERR_STATE, "mpm_set_hand: the batched step applies no hand push");
+  }
+  if (!active && !s->hand_on) return MPM_OK;
+  HIPCHK(hipSetDevice(s->cfg.device));
+  int rc = ensure_interact(s);
+  if (rc) return rc;
+  HandDev d{};
+  if (active) {
+    d.px = hand->pos[0];
+    d.py = hand->pos[1];
+    d.pz = hand->pos[2];
+    d.radius = hand->radius;
+    d.vx = hand->vel[0];
+    d.vy = hand->vel[1];
+    d.vz = hand->vel[2];
+    d.strength = hand->strength;
+    d.ex = hand->eye[0];
+    d.ey = hand->eye[1];
+    d.ez = hand->eye[2];
+    d.on = 1u;
+    hand_box(s->cfg, *hand, d);
+  }
+  // ordered after the steps already enqueued, before the next ones
+  HIPCHK(hipMemcpyAsync(s->hand_dev, &d, sizeof d, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (active && !s->hand_on) {
+    scene_drop_graph(s);  // the step gains the hand kernel
+    s->hand_on = true;
+    s->version++;
+  } else if (!active) {
+    // steps captured with the kernel stay correct: it reads `on` = 0
+    s->hand_on = false;
+  }
+  return MPM_OK;
+}
+
+static Ray make_ray(const float *o, const float *d, float radius) {
+  Ray r{};
+  r.ox = o[0];
+  r.oy = o[1];
+  r.oz = o[2];
+  r.dx = d[0];
+  r.dy = d[1];
+  r.dz = d[2];
+  r.radius = radius;
+  return r;
+}
+
+static int check_ray(const float *o, const float *d, float radius, const char *what) {
+  if (!o || !d) return failf(MPM_ERR_INVALID, "%s: origin or dir is NULL", what);
+  for (int a = 0; a < 3; a++)
+    if (!std::isfinite(o[a]) || !std::isfinite(d[a]))
+      return failf(MPM_ERR_INVALID, "%s: origin and dir must be finite", what);
+  if (!(radius >= 0.0f) || !std::isfinite(radius))
+    return failf(MPM_ERR_INVALID, "%s: radius must be finite and >= 0", what);
+  return MPM_OK;
+}
+
+int scene_delete_near(mpm_scene *s, const float *origin, const float *dir, float radius,
+                      uint32_t *remaining) {
+  int rc = check_ray(origin, dir, radius, "mpm_delete_near");
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(s->cfg.device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if ((rc = ensure_interact(s))) return rc;
+  const uint32_t n = s->n, nw = (n + 31) / 32;
+  uint32_t kept = 0;
+  if (n) {
+    launch_delete_mark_move(pstate(s, s->cur), pstate(s, s->cur ^ 1), n,
+                            make_ray(origin, dir, radius), s->ibits, s->iprefix, s->iscratch,
+                            s->itotal, nw, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&kept, s->itotal, 4, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    // the survivors are in reset order in the other buffer: re-sort next step
+    scene_drop_graph(s);
+    s->cur ^= 1;
+    s->n = kept;
+    s->since_sort = 0;
+    s->sorted_once = false;
+    s->version++;
+    if (!kept && s->batch) {
+      // batched steps skip scenes without particles: leave the grid as the
+      // next (empty) step would
+      if ((rc = zero_grids(s))) return rc;
+      HIPCHK(hipStreamSynchronize(s->stream));
+    }
+  }
+  if (remaining) *remaining = kept;
+  return MPM_OK;
+}
+
+int scene_raycast(mpm_scene *s, const float *origin, const float *dir, float radius, float *t,
+                  int *hit) {
+  int rc = check_ray(origin, dir, radius, "mpm_raycast");
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(s->cfg.device));
+  if ((rc = ensure_interact(s))) return rc;
+  launch_raycast(pstate(s, s->cur), s->n, make_ray(origin, dir, radius), s->ihit, s->stream);
+  HIPCHK(hipGetLastError());
+  uint32_t bits = 0xffffffffu;
+  HIPCHK(hipMemcpyAsync(&bits, s->ihit, 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const int h = bits != 0xffffffffu;
+  if (hit) *hit = h;
+  if (t) {
+    float f = 0.0f;
+    if (h) std::memcpy(&f, &bits, 4);
+    *t = f;
+  }
+  return MPM_OK;
+}
+
+int scene_append(mpm_scene *s, uint32_t k, const float *pos3, const float *vel3) {
+  const mpm_config &c = s->cfg;
+  if (!k) return MPM_OK;
+  if (!pos3) return failf(MPM_ERR_INVALID, "mpm_append_particles: pos3 is NULL");
+  if ((uint64_t)s->n + k > c.capacity)
+    return failf(MPM_ERR_CAPACITY, "append of %u particles to %u exceeds capacity %u", k, s->n,
+                 c.capacity);
+  const float lim[3] = {(float)c.grid_x - 1.0f, (float)c.grid_y - 1.0f, (float)c.grid_z - 1.0f};
+  for (uint64_t i = 0; i < (uint64_t)k * 3; i++) {
+    const float x = pos3[i];
+    if (!(x >= 1.0f && x < lim[i % 3]))
+      return failf(MPM_ERR_RANGE, "appended particle %llu: coordinate %u = %g outside [1, %g)",
+                   (unsigned long long)(i / 3), (unsigned)(i % 3), (double)x,
+                   (double)lim[i % 3]);
+  }
+  if (vel3)
+    for (uint64_t i = 0; i < (uint64_t)k * 3; i++)
+      if (!std::isfinite(vel3[i]))
+        return failf(MPM_ERR_RANGE, "appended particle %llu: velocity not finite",
+                     (unsigned long long)(i / 3));
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  int rc = ensure_stage(s, (size_t)k * 4 * (vel3 ? 6 : 3));
+  if (rc) return rc;
+  float *dp = (float *)s->stage, *dv = vel3 ? dp + (size_t)k * 3 : nullptr;
+  HIPCHK(hipMemcpyAsync(dp, pos3, (size_t)k * 12, hipMemcpyHostToDevice, s->stream));
+  if (vel3) HIPCHK(hipMemcpyAsync(dv, vel3, (size_t)k * 12, hipMemcpyHostToDevice, s->stream));
+  launch_import_particles(pstate(s, s->cur), k, dp, dv, nullptr, nullptr, s->stream, s->n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  // the new particles sit after the sorted ones: re-sort next step
+  scene_drop_graph(s);
+  s->n += k;
+  s->since_sort = 0;
+  s->sorted_once = false;
+  s->version++;
   return MPM_OK;
 }
 

@@ -395,6 +395,108 @@ static napi_value js_set_walls(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* Three numbers from a JavaScript array or TypedArray. */
+static int vec3_of(napi_env env, napi_value v, float out[3]) {
+  bool is_arr = false, is_ta = false;
+  napi_is_array(env, v, &is_arr);
+  napi_is_typedarray(env, v, &is_ta);
+  if (!is_arr && !is_ta) return 0;
+  for (uint32_t i = 0; i < 3; i++) {
+    napi_value e;
+    double x;
+    if (napi_get_element(env, v, i, &e) != napi_ok) return 0;
+    if (napi_get_value_double(env, e, &x) != napi_ok) return 0;
+    out[i] = (float)x;
+  }
+  return 1;
+}
+
+/* setHand(handle, pos, vel, radius, strength, active, eye) (updateHand,
+ * fluid-gpu.js:479-529); setHand(handle, null) switches the push off. */
+static napi_value js_set_hand(napi_env env, napi_callback_info info) {
+  napi_value a[7];
+  if (!get_args(env, info, 7, a)) return NULL;
+  mpm_scene *s = scene_of(env, a[0]);
+  if (!s) return NULL;
+  if (is_nullish(env, a[1])) {
+    MPM_CALL(env, mpm_set_hand(s, NULL));
+    return NULL;
+  }
+  mpm_hand h;
+  memset(&h, 0, sizeof h);
+  if (!vec3_of(env, a[1], h.pos) || !vec3_of(env, a[2], h.vel) || !vec3_of(env, a[6], h.eye)) {
+    napi_throw_type_error(env, NULL, "setHand: pos, vel and eye must hold three numbers");
+    return NULL;
+  }
+  h.radius = (float)f64_of(env, a[3]);
+  h.strength = (float)f64_of(env, a[4]);
+  bool on = false;
+  napi_coerce_to_bool(env, a[5], &a[5]);
+  napi_get_value_bool(env, a[5], &on);
+  h.active = on ? 1u : 0u;
+  MPM_CALL(env, mpm_set_hand(s, &h));
+  return NULL;
+}
+
+/* deleteNear(handle, origin, dir, radius) -> particles left (fluid-gpu.js:547-575) */
+static napi_value js_delete_near(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!get_args(env, info, 4, a)) return NULL;
+  mpm_scene *s = scene_of(env, a[0]);
+  if (!s) return NULL;
+  float o[3], d[3];
+  if (!vec3_of(env, a[1], o) || !vec3_of(env, a[2], d)) {
+    napi_throw_type_error(env, NULL, "deleteNear: origin and dir must hold three numbers");
+    return NULL;
+  }
+  uint32_t left = 0;
+  MPM_CALL(env, mpm_delete_near(s, o, d, (float)f64_of(env, a[3]), &left));
+  napi_value v;
+  napi_create_uint32(env, left, &v);
+  return v;
+}
+
+/* raycast(handle, origin, dir, radius) -> t or null (raycastFluid + pollRaycast,
+ * fluid-gpu.js:597-638) */
+static napi_value js_raycast(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!get_args(env, info, 4, a)) return NULL;
+  mpm_scene *s = scene_of(env, a[0]);
+  if (!s) return NULL;
+  float o[3], d[3];
+  if (!vec3_of(env, a[1], o) || !vec3_of(env, a[2], d)) {
+    napi_throw_type_error(env, NULL, "raycast: origin and dir must hold three numbers");
+    return NULL;
+  }
+  float t = 0.0f;
+  int hit = 0;
+  MPM_CALL(env, mpm_raycast(s, o, d, (float)f64_of(env, a[3]), &t, &hit));
+  napi_value v;
+  if (hit) napi_create_double(env, (double)t, &v);
+  else napi_get_null(env, &v);
+  return v;
+}
+
+/* append(handle, pos Float32Array(3k), vel Float32Array(3k) | null) -> k
+ * (spawnParticles' upload, fluid-gpu.js:145-180) */
+static napi_value js_append(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return NULL;
+  mpm_scene *s = scene_of(env, a[0]);
+  if (!s) return NULL;
+  float *pos, *vel;
+  size_t np, nv;
+  if (!f32_array(env, a[1], &pos, &np) || !f32_array(env, a[2], &vel, &nv) || !pos ||
+      np % 3 || (vel && nv < np)) {
+    napi_throw_type_error(env, NULL, "append: expected Float32Array(3k) pos and vel (or null)");
+    return NULL;
+  }
+  MPM_CALL(env, mpm_append_particles(s, (uint32_t)(np / 3), pos, vel));
+  napi_value v;
+  napi_create_uint32(env, (uint32_t)(np / 3), &v);
+  return v;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
   (void)info;
   napi_value v;
@@ -419,6 +521,10 @@ static napi_value init(napi_env env, napi_value exports) {
       {"getConfig", js_get_config},
       {"setSeed", js_set_seed},
       {"setWalls", js_set_walls},
+      {"setHand", js_set_hand},
+      {"deleteNear", js_delete_near},
+      {"raycast", js_raycast},
+      {"append", js_append},
   };
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;

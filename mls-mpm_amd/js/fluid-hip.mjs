@@ -14,6 +14,12 @@
 //   - Like the reference (errors go to WebGPU error scopes, methods return
 //     bool), simFrame() returns false on a device error and reports it to
 //     this.onerror (if set) and this.lastError; initGPU() throws.
+//   - deleteNear() and raycastFluid() accept the reference's leading command
+//     encoder argument and ignore it; they complete before returning, so
+//     active_particle_num and lastRayHit are current at once and
+//     pollDelete() / pollRaycast() have nothing left to do.
+//   - deleteNear() keeps the survivors in their order and carries their
+//     affine momentum; spawned particles start with zero affine momentum.
 import { createRequire } from 'module';
 
 const require = createRequire(import.meta.url);
@@ -65,7 +71,11 @@ export class FluidHIP {
         this._wallF32 = new Float32Array(4 + MAX_WALLS * 20);
         this._wallSent = null;
 
-        this.random = Math.random;  // jitter source of fillBlock (seedable)
+        this.random = Math.random;  // jitter source of fillBlock / spawnParticles (seedable)
+        this.handActive = false;
+        this._hand = null;          // last updateHand arguments, re-sent on a rebuild
+        this.lastRayHit = null;
+        this._deletePending = false;
         this.device = null;
         this.onerror = null;
         this.lastError = null;
@@ -229,6 +239,7 @@ export class FluidHIP {
         this._sent = cfg;
         this._wallSent = null;
         const n = this.active_particle_num;
+        const hand = this._hand;
         if (state) {
             addon.reset(this._scene, state.pos, state.vel, state.C, this.particle_mass.subarray(0, n));
         } else {
@@ -236,6 +247,7 @@ export class FluidHIP {
                         this.particle_vel.subarray(0, 3 * n),
                         this.particle_affine.subarray(0, 9 * n), this.particle_mass.subarray(0, n));
         }
+        if (hand) this.updateHand(...hand);
     }
 
     // A diffuse pool attached after initGPU: the scene is rebuilt with it,
@@ -314,6 +326,77 @@ export class FluidHIP {
         addon.readParticles(this._scene, null, null, null, o);
         return o;
     }
+
+    // -- interaction (fluid-gpu.js:145-180, 479-638) ------------------------
+
+    // Spray `count` particles (at most the free capacity) uniformly in a ball
+    // of `radius` around (cx, cy, cz), clamped to the hard bounds, with
+    // velocity vel + sprayVel along the offset.  Returns how many were added.
+    spawnParticles(cx, cy, cz, count, radius, vel = [0, 0, 0], sprayVel = 3.0) {
+        if (this._deletePending) return 0;
+        const start = this.active_particle_num;
+        const n = Math.min(count, this.particle_num - start);
+        if (n <= 0) return 0;
+        const pos = new Float32Array(3 * n), v = new Float32Array(3 * n);
+        const lo = this.HARD_MIN, hi = [this.HARD_MAX_X, this.HARD_MAX_Y, this.HARD_MAX_Z];
+        const c = [cx, cy, cz];
+        for (let i = 0; i < n; i++) {
+            const r = radius * Math.cbrt(this.random());
+            const theta = this.random() * Math.PI * 2;
+            const cphi = this.random() * 2 - 1;
+            const sphi = Math.sqrt(1 - cphi * cphi);
+            const d = [r * sphi * Math.cos(theta), r * sphi * Math.sin(theta), r * cphi];
+            const rl = Math.hypot(d[0], d[1], d[2]) || 1;
+            for (let a = 0; a < 3; a++) {
+                pos[3 * i + a] = Math.min(hi[a], Math.max(lo, c[a] + d[a]));
+                v[3 * i + a] = vel[a] + d[a] / rl * sprayVel;
+            }
+        }
+        if (this._scene) {
+            addon.append(this._scene, pos, v);
+        } else {
+            this.particle_pos.set(pos, 3 * start);
+            this.particle_vel.set(v, 3 * start);
+            this.particle_affine.fill(0, 9 * start, 9 * (start + n));
+            this.particle_mass.fill(1, start, start + n);
+        }
+        this.active_particle_num = start + n;
+        return n;
+    }
+
+    // The push of cells within `radius` of the line from `eye` through `pos`:
+    // vel * (1 - d / radius)^2 * strength, added after the grid update.
+    updateHand(pos, vel, radius, strength, active, eye) {
+        this.handActive = !!active;
+        this._hand = [pos, vel, radius, strength, active, eye];
+        if (!this._scene) return;
+        if (active) addon.setHand(this._scene, pos, vel, radius, strength, true, eye);
+        else addon.setHand(this._scene, null);
+    }
+
+    // Removes the particles within `radius` of the line through `origin`
+    // along `dir`.  Returns true (the removal is complete on return).
+    deleteNear(...args) {
+        const [origin, dir, radius] = args.length > 3 ? args.slice(1) : args;
+        if (!this._scene) this.initGPU(this.device === null ? 0 : this.device);
+        this.active_particle_num = addon.deleteNear(this._scene, origin, dir, radius);
+        return true;
+    }
+
+    pollDelete() {}
+
+    // Where the ray first meets the fluid within `radius` of its axis:
+    // this.lastRayHit = [x, y, z] or null.  Returns true.
+    raycastFluid(...args) {
+        const [origin, dir, radius] = args.length > 3 ? args.slice(1) : args;
+        if (!this._scene) this.initGPU(this.device === null ? 0 : this.device);
+        const t = addon.raycast(this._scene, origin, dir, radius);
+        this.lastRayHit = t === null ? null
+            : [origin[0] + t * dir[0], origin[1] + t * dir[1], origin[2] + t * dir[2]];
+        return true;
+    }
+
+    pollRaycast() {}
 
     stepsTaken() {
         return this._scene ? addon.counts(this._scene).steps : 0;

@@ -24,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["mpm_kernels.hip", "mpm_tiled.hip", "mpm_scene.hip", "mpm_capi.hip",
-               "mpm_batch.hip", "mpm_server.hip"]
+               "mpm_batch.hip", "mpm_server.hip", "mpm_interact.hip"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",

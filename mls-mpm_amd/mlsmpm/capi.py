@@ -105,6 +105,15 @@ class RequestInfo(C.Structure):
                 ("t_admit", C.c_double), ("t_done", C.c_double)]
 
 
+class Hand(C.Structure):
+    """mpm_hand: the push of FluidGPU.updateHand (fluid-gpu.js:479-529)."""
+    _fields_ = [("pos", C.c_float * 3), ("radius", C.c_float), ("vel", C.c_float * 3),
+                ("active", C.c_uint32), ("strength", C.c_float), ("eye", C.c_float * 3)]
+
+
+assert C.sizeof(Hand) == 48
+
+
 # Every entry point declared in include/mlsmpm.h, with its argument types.
 _VP = C.c_void_p
 _SIGNATURES = {
@@ -127,6 +136,10 @@ _SIGNATURES = {
     "mpm_get_config": ([_VP, _VP], C.c_int),
     "mpm_set_seed": ([_VP, C.c_uint32], C.c_int),
     "mpm_set_walls": ([_VP, _VP], C.c_int),
+    "mpm_set_hand": ([_VP, _VP], C.c_int),
+    "mpm_delete_near": ([_VP, _VP, _VP, C.c_float, _VP], C.c_int),
+    "mpm_raycast": ([_VP, _VP, _VP, C.c_float, _VP, _VP], C.c_int),
+    "mpm_append_particles": ([_VP, C.c_uint32, _VP, _VP], C.c_int),
     "mpm_get_stream": ([_VP, _VP], C.c_int),
     "mpm_enable_timing": ([_VP, C.c_int], C.c_int),
     "mpm_stage_times": ([_VP, _VP], C.c_int),

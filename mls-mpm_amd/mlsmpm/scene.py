@@ -149,6 +149,55 @@ class Scene:
         w = np.ascontiguousarray(np.asarray(wallp, dtype=np.float32).reshape(84))
         check(load().mpm_set_walls(self._h, _p(w)))
 
+    # -- interaction (fluid-gpu.js:145-180, 479-638) ----------------------
+    def set_hand(self, pos, vel, radius: float, strength: float, active: bool = True, eye=None):
+        """updateHand: push the fluid along the line from `eye` through `pos`
+        (cells within `radius` of it get vel * (1 - d/radius)^2 * strength
+        added after the grid update).  active=False switches it off."""
+        h = capi.Hand()
+        h.pos[:] = [float(x) for x in pos]
+        h.vel[:] = [float(x) for x in vel]
+        h.eye[:] = [float(x) for x in (eye if eye is not None else (0.0, 0.0, 0.0))]
+        h.radius = float(radius)
+        h.strength = float(strength)
+        h.active = 1 if active else 0
+        check(load().mpm_set_hand(self._h, C.byref(h)))
+
+    def clear_hand(self):
+        check(load().mpm_set_hand(self._h, None))
+
+    @staticmethod
+    def _ray(origin, direction):
+        o = np.ascontiguousarray(np.asarray(origin, np.float32).reshape(3))
+        d = np.ascontiguousarray(np.asarray(direction, np.float32).reshape(3))
+        return o, d
+
+    def delete_near(self, origin, direction, radius: float) -> int:
+        """deleteNear: removes the particles within `radius` of the ray's
+        axis (direction taken as given, normally unit length); returns the
+        number left.  Survivors keep their relative order."""
+        o, d = self._ray(origin, direction)
+        left = C.c_uint32()
+        check(load().mpm_delete_near(self._h, _p(o), _p(d), float(radius), C.byref(left)))
+        return left.value
+
+    def raycast(self, origin, direction, radius: float):
+        """raycastFluid: (hit, t) with t the smallest positive distance along
+        the ray among particles within `radius` of its axis."""
+        o, d = self._ray(origin, direction)
+        t = C.c_float()
+        hit = C.c_int()
+        check(load().mpm_raycast(self._h, _p(o), _p(d), float(radius), C.byref(t), C.byref(hit)))
+        return bool(hit.value), float(t.value)
+
+    def append(self, pos, vel=None):
+        """spawnParticles' upload: new particles after the current ones, unit
+        mass, zero affine momentum."""
+        pos = _f32(pos, (-1, 3))
+        k = pos.shape[0]
+        vel = _f32(vel, (k, 3))
+        check(load().mpm_append_particles(self._h, k, _p(pos), _p(vel)))
+
     def enable_timing(self, on: bool = True):
         check(load().mpm_enable_timing(self._h, 1 if on else 0))
 

@@ -663,6 +663,116 @@ void orc_diffuse_advect(const orc_params *P, const orc_adv *adv, orc_drec *pool,
     }
 }
 
+/* ---------------------------------------------------------------------------
+ * Interaction.  The hand box is host arithmetic (JS numbers are doubles):
+ * FluidGPU.updateHand (fluid-gpu.js:479-527) with lineBoxIntersect
+ * (math.js:149-164). */
+void orc_hand_bbox(const orc_params *P, orc_hand *h) {
+    for (int a = 0; a < 3; a++) {
+        h->bbox_min[a] = 0;
+        h->bbox_max[a] = -1;
+    }
+    if (!h->is_on) return;
+    const double pos[3] = {h->px, h->py, h->pz}, eye[3] = {h->ex, h->ey, h->ez};
+    const double g[3] = {(double)P->grid_x, (double)P->grid_y, (double)P->grid_z};
+    double d[3] = {pos[0] - eye[0], pos[1] - eye[1], pos[2] - eye[2]};
+    double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (dl == 0.0) dl = 1.0;
+    for (int a = 0; a < 3; a++) d[a] /= dl;
+    const double R = h->radius;
+    double tmin = -INFINITY, tmax = INFINITY;
+    for (int a = 0; a < 3; a++) {
+        const double lo = -R, hi = g[a] + R;
+        if (fabs(d[a]) < 1e-9) {
+            if (pos[a] < lo || pos[a] > hi) return;
+            continue;
+        }
+        double t1 = (lo - pos[a]) / d[a], t2 = (hi - pos[a]) / d[a];
+        if (t1 > t2) {
+            const double tt = t1;
+            t1 = t2;
+            t2 = tt;
+        }
+        tmin = fmax(tmin, t1);
+        tmax = fmin(tmax, t2);
+        if (tmin > tmax) return;
+    }
+    for (int a = 0; a < 3; a++) {
+        const double p0 = pos[a] + tmin * d[a], p1 = pos[a] + tmax * d[a];
+        const double lo = floor(fmin(p0, p1) - R), hi = ceil(fmax(p0, p1) + R);
+        h->bbox_min[a] = (int32_t)fmax(0.0, lo);
+        h->bbox_max[a] = (int32_t)fmin(g[a] - 1.0, hi);
+    }
+}
+
+/* WGSL_APPLY_HAND (shaders.js:410-432): inside the box, cells within `radius`
+ * of the line through the hand along the eye ray get `vel * (1 - d/r)^2 *
+ * strength` added to their (post-update) velocity. */
+void orc_apply_hand(const orc_params *P, const orc_hand *h, real *cell_v) {
+    if (!h->is_on) return;
+    const int X = (int)P->grid_x, Y = (int)P->grid_y, Z = (int)P->grid_z;
+    const v3 hp = mk3((real)h->px, (real)h->py, (real)h->pz);
+    const v3 e0 = mk3(hp.x - (real)h->ex, hp.y - (real)h->ey, hp.z - (real)h->ez);
+    const real len = SQRTR(dot3(e0, e0));
+    const v3 rd = mk3(e0.x / len, e0.y / len, e0.z / len);
+    const real radius = (real)h->radius;
+    for (int cz = h->bbox_min[2]; cz <= h->bbox_max[2]; cz++)
+        for (int cy = h->bbox_min[1]; cy <= h->bbox_max[1]; cy++)
+            for (int cx = h->bbox_min[0]; cx <= h->bbox_max[0]; cx++) {
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= X || cy >= Y || cz >= Z) continue;
+                const size_t idx = ((size_t)cz * Y + cy) * X + cx;
+                const v3 to = mk3(((real)cx + K(0.5)) - hp.x, ((real)cy + K(0.5)) - hp.y,
+                                  ((real)cz + K(0.5)) - hp.z);
+                const real t0 = dot3(to, rd);
+                const v3 perp = mk3(to.x - t0 * rd.x, to.y - t0 * rd.y, to.z - t0 * rd.z);
+                const real d = SQRTR(dot3(perp, perp));
+                if (d >= radius) continue;
+                const real t = K(1.0) - d / radius;
+                const real w = (t * t) * (real)h->strength;
+                cell_v[3 * idx + 0] += (real)h->vx * w;
+                cell_v[3 * idx + 1] += (real)h->vy * w;
+                cell_v[3 * idx + 2] += (real)h->vz * w;
+            }
+}
+
+/* ray_perp_dist (shaders.js:80-84). */
+real orc_ray_perp_dist(const real *p, const orc_ray *r) {
+    const v3 to = mk3(p[0] - (real)r->ox, p[1] - (real)r->oy, p[2] - (real)r->oz);
+    const v3 d = mk3((real)r->dx, (real)r->dy, (real)r->dz);
+    const real t = dot3(to, d);
+    const v3 perp = mk3(to.x - t * d.x, to.y - t * d.y, to.z - t * d.z);
+    return SQRTR(dot3(perp, perp));
+}
+
+/* WGSL_RAYCAST (shaders.js:557-577): the smallest positive distance along the
+ * ray among particles within `radius` of its axis.  Returns whether any. */
+int orc_raycast(const real *pos4, uint32_t n, const orc_ray *r, real *t_out) {
+    int hit = 0;
+    real best = K(0.0);
+    for (uint32_t p = 0; p < n; p++) {
+        const real *q = pos4 + 4 * (size_t)p;
+        const v3 to = mk3(q[0] - (real)r->ox, q[1] - (real)r->oy, q[2] - (real)r->oz);
+        const real t = dot3(to, mk3((real)r->dx, (real)r->dy, (real)r->dz));
+        if (t <= K(0.0)) continue;
+        if (orc_ray_perp_dist(q, r) >= (real)r->radius) continue;
+        if (!hit || t < best) best = t;
+        hit = 1;
+    }
+    *t_out = best;
+    return hit;
+}
+
+/* WGSL_COMPACT (shaders.js:521-547): keep[p] = 1 for particles outside the
+ * cylinder.  Returns the number kept. */
+uint32_t orc_delete_mask(const real *pos4, uint32_t n, const orc_ray *r, uint8_t *keep) {
+    uint32_t k = 0;
+    for (uint32_t p = 0; p < n; p++) {
+        keep[p] = orc_ray_perp_dist(pos4 + 4 * (size_t)p, r) < (real)r->radius ? 0 : 1;
+        k += keep[p];
+    }
+    return k;
+}
+
 /* FluidGPU.simFrame with SUBSTEPS = 1 (fluid-gpu.js:648-684) followed by
  * DiffuseGPU.step (diffuse-gpu.js:148-166): seed advances before generate. */
 void orc_step(const orc_params *P, orc_state *S, uint32_t nsteps) {
@@ -673,6 +783,7 @@ void orc_step(const orc_params *P, orc_state *S, uint32_t nsteps) {
         orc_decode_mass(P, S->cell_mass, S->cell_mass_f);
         orc_p2g_mom(P, S->vel4, S->affine, S->mass, S->cell_mass_f, S->cell_mv, S->density, S->bw);
         orc_update_grid(P, S->cell_mv, S->cell_mass, S->cell_v, S->wallp);
+        if (S->hand) orc_apply_hand(P, S->hand, S->cell_v);
         orc_g2p(P, S->pos4, S->vel4, S->affine, S->cell_v, S->bw, S->wallp);
         if (S->pool) {
             S->gen.seed = S->gen.seed + 1u;

@@ -73,6 +73,23 @@ typedef struct {
 #define ORC_MAX_WALLS 4
 #define ORC_WALLP_FLOATS (4 + ORC_MAX_WALLS * 20)
 
+/* HandParams uniform, 80 bytes (shaders.js:394-402), filled by
+ * FluidGPU.updateHand (fluid-gpu.js:479-527) including the cell box of the
+ * push cylinder inside the grid (bbox_min > bbox_max: nothing dispatched). */
+typedef struct {
+    float px, py, pz, radius;
+    float vx, vy, vz;
+    uint32_t is_on;
+    float strength, ex, ey, ez;
+    int32_t bbox_min[3], bbox_max[3];
+    int32_t pad0, pad1;
+} orc_hand;
+
+/* RayParams (shaders.js:78): ray origin, cylinder radius, direction. */
+typedef struct {
+    float ox, oy, oz, radius, dx, dy, dz, pad;
+} orc_ray;
+
 /* Whole-scene state for orc_step: caller-owned buffers. */
 typedef struct {
     orc_real *pos4, *vel4, *affine, *mass, *density; /* 4n, 4n, 9n, n, n */
@@ -86,6 +103,7 @@ typedef struct {
     int32_t *top;
     orc_gen gen; /* gen.seed is advanced by orc_step like DiffuseGPU._seed */
     orc_adv adv;
+    const orc_hand *hand; /* NULL: no hand push */
 } orc_state;
 
 #ifdef __cplusplus
@@ -114,6 +132,11 @@ void orc_diffuse_generate(const orc_params *P, const orc_gen *gen, const orc_rea
 void orc_diffuse_advect(const orc_params *P, const orc_adv *adv, orc_drec *pool,
                         const orc_real *cell_v, uint32_t *free_list, int32_t *top,
                         const float *wallp);
+void orc_hand_bbox(const orc_params *P, orc_hand *h);
+void orc_apply_hand(const orc_params *P, const orc_hand *h, orc_real *cell_v);
+orc_real orc_ray_perp_dist(const orc_real *pos4, const orc_ray *r);
+int orc_raycast(const orc_real *pos4, uint32_t n, const orc_ray *r, orc_real *t);
+uint32_t orc_delete_mask(const orc_real *pos4, uint32_t n, const orc_ray *r, uint8_t *keep);
 void orc_step(const orc_params *P, orc_state *S, uint32_t nsteps);
 #ifdef __cplusplus
 }

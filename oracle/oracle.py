@@ -108,6 +108,14 @@ class _Lib:
         L.orc_diffuse_rand.argtypes = [C.c_uint32]
         L.orc_diffuse_rand.restype = C.c_float
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_hand_bbox.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_apply_hand.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_ray_perp_dist.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_ray_perp_dist.restype = ct
+        L.orc_raycast.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+        L.orc_raycast.restype = C.c_int
+        L.orc_delete_mask.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+        L.orc_delete_mask.restype = C.c_uint32
         self.bw_dtype = np.dtype([("b", np.int32, 3), ("f", self.real, 3), ("w", self.real, (3, 3))],
                                  align=True)
         self.drec_dtype = np.dtype([("pos", self.real, 4), ("vel", self.real, 4), ("ptype", np.uint32),
@@ -133,13 +141,36 @@ def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+class Hand(C.Structure):
+    """HandParams (shaders.js:394-402), box filled by orc_hand_bbox."""
+    _fields_ = [("px", C.c_float), ("py", C.c_float), ("pz", C.c_float), ("radius", C.c_float),
+                ("vx", C.c_float), ("vy", C.c_float), ("vz", C.c_float), ("is_on", C.c_uint32),
+                ("strength", C.c_float), ("ex", C.c_float), ("ey", C.c_float), ("ez", C.c_float),
+                ("bbox_min", C.c_int32 * 3), ("bbox_max", C.c_int32 * 3),
+                ("pad0", C.c_int32), ("pad1", C.c_int32)]
+
+
+class Ray(C.Structure):
+    """RayParams (shaders.js:78)."""
+    _fields_ = [("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("radius", C.c_float),
+                ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float), ("pad", C.c_float)]
+
+
+def make_ray(origin, direction, radius) -> Ray:
+    r = Ray()
+    r.ox, r.oy, r.oz = (float(v) for v in origin)
+    r.dx, r.dy, r.dz = (float(v) for v in direction)
+    r.radius = float(radius)
+    return r
+
+
 class _State(C.Structure):
     _fields_ = [
         ("pos4", C.c_void_p), ("vel4", C.c_void_p), ("affine", C.c_void_p), ("mass", C.c_void_p),
         ("density", C.c_void_p), ("bw", C.c_void_p), ("cell_mass", C.c_void_p),
         ("cell_mv", C.c_void_p), ("cell_mass_f", C.c_void_p), ("cell_v", C.c_void_p),
         ("wallp", C.c_void_p), ("pool", C.c_void_p), ("free_list", C.c_void_p),
-        ("top", C.c_void_p), ("gen", Gen), ("adv", Adv),
+        ("top", C.c_void_p), ("gen", Gen), ("adv", Adv), ("hand", C.c_void_p),
     ]
 
 
@@ -155,6 +186,7 @@ class OracleScene:
     walls: np.ndarray | None = None
     gen: Gen = field(default=None)
     adv: Adv = field(default=None)
+    hand: Hand = field(default=None)
 
     def __post_init__(self):
         gx, gy, gz = self.grid
@@ -213,7 +245,59 @@ class OracleScene:
         s.wallp = _ptr(self.walls)
         s.pool, s.free_list, s.top = _ptr(self.pool), _ptr(self.free_list), _ptr(self.top)
         s.gen, s.adv = self.gen, self.adv
+        s.hand = C.cast(C.pointer(self.hand), C.c_void_p) if self.hand is not None else None
         return s
+
+    # --- interaction (fluid-gpu.js:145-180, 479-638) --------------------------
+    def set_hand(self, pos, vel, radius, strength, active, eye):
+        """FluidGPU.updateHand: the push applies in every later step."""
+        h = Hand()
+        h.px, h.py, h.pz = (float(v) for v in pos)
+        h.vx, h.vy, h.vz = (float(v) for v in vel)
+        h.ex, h.ey, h.ez = (float(v) for v in eye)
+        h.radius, h.strength, h.is_on = float(radius), float(strength), 1 if active else 0
+        self.L.lib.orc_hand_bbox(C.byref(self.params), C.byref(h))
+        self.hand = h
+
+    def raycast(self, origin, direction, radius):
+        """raycastFluid: (hit, t) of the nearest particle near the ray."""
+        t = np.zeros(1, self.L.real)
+        hit = self.L.lib.orc_raycast(_ptr(self.pos4), self.n,
+                                     C.byref(make_ray(origin, direction, radius)), _ptr(t))
+        return bool(hit), float(t[0])
+
+    def delete_near(self, origin, direction, radius) -> int:
+        """deleteNear: drop the particles within `radius` of the line; the
+        survivors keep their order (and their C)."""
+        keep = np.zeros(self.n, np.uint8)
+        k = self.L.lib.orc_delete_mask(_ptr(self.pos4), self.n,
+                                       C.byref(make_ray(origin, direction, radius)), _ptr(keep))
+        m = keep.astype(bool)
+        for name in ("pos4", "vel4", "affine", "mass", "density", "bw"):
+            setattr(self, name, np.ascontiguousarray(getattr(self, name)[m]))
+        self.n = int(k)
+        self.params.particle_num = self.n
+        return self.n
+
+    def append(self, pos, vel=None):
+        """spawnParticles' upload: particles added after the current ones with
+        unit mass and zero C."""
+        R = self.L.real
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        k = pos.shape[0]
+        p4 = np.zeros((k, 4), R)
+        p4[:, :3] = pos
+        v4 = np.zeros((k, 4), R)
+        if vel is not None:
+            v4[:, :3] = np.asarray(vel, dtype=np.float64).reshape(-1, 3)
+        self.pos4 = np.concatenate([self.pos4, p4])
+        self.vel4 = np.concatenate([self.vel4, v4])
+        self.affine = np.concatenate([self.affine, np.zeros((k, 9), R)])
+        self.mass = np.concatenate([self.mass, np.ones(k, R)])
+        self.density = np.concatenate([self.density, np.zeros(k, R)])
+        self.bw = np.concatenate([self.bw, np.zeros(k, self.L.bw_dtype)])
+        self.n += k
+        self.params.particle_num = self.n
 
     def step(self, nsteps: int = 1, threads: int = 0):
         if threads:

@@ -8,6 +8,8 @@ term, a wrong sign or index, a transposed operand) fails a check.  The pins
 compute their expectations independently (float64, textbook identities);
 they do not retype the oracle's f32 expressions.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -437,3 +439,61 @@ def test_reference_float_literals_round_once():
             "0.01", "8.0", "1.2"]
     for t in lits:
         assert np.float32(t) == np.float32(np.float64(t)), t
+
+
+# --- interaction ops (fluid-gpu.js:145-180, 479-638; shaders.js:394-433, 521-577)
+
+def test_raycast_nearest_positive_hit_on_the_axis():
+    """Particles on the axis at t = 5, 3 and -2, one off-axis beyond the
+    radius at t = 1: the hit is t = 3 exactly (behind the origin and outside
+    the cylinder do not count)."""
+    s = O.OracleScene(grid=(32, 32, 32))
+    d = np.array([0.6, 0.0, 0.8])
+    o = np.array([4.0, 10.0, 4.0])
+    pts = [o + 5 * d, o + 3 * d, o - 2 * d, o + 1 * d + np.array([0.0, 2.0, 0.0])]
+    s.reset(np.array(pts, np.float32))
+    hit, t = s.raycast(o, d, 1.0)
+    assert hit and abs(t - 3.0) < 2e-6
+    s.reset(np.array([o - 2 * d], np.float32))
+    assert s.raycast(o, d, 1.0)[0] is False
+
+
+def test_delete_near_removes_the_cylinder_and_keeps_order():
+    s = O.OracleScene(grid=(32, 32, 32))
+    o, d = np.array([16.0, 16.0, 2.0]), np.array([0.0, 0.0, 1.0])
+    # perpendicular distances 0.5 (removed), 1.5, 0.99 (removed), 1.0 (kept: the
+    # test is dist < radius), 3 -- at assorted depths along the line
+    pos = np.array([[16.5, 16, 5], [17.5, 16, 9], [16, 16.99, 20], [16, 17, 12], [13, 16, 30]],
+                   np.float32)
+    vel = np.arange(15, dtype=np.float32).reshape(5, 3)
+    s.reset(pos, vel)
+    assert s.delete_near(o, d, 1.0) == 3
+    np.testing.assert_array_equal(s.pos4[:, :3], pos[[1, 3, 4]])
+    np.testing.assert_array_equal(s.vel4[:, :3], vel[[1, 3, 4]])
+
+
+def test_hand_push_profile_along_the_cylinder():
+    """A push at cell centre (10.5, 8.5, 12.5) seen along +z: the cell on the
+    axis gets vel * strength, a cell one cell off the axis (any depth) gets
+    vel * (1 - 1/2)^2 * strength, cells at >= the radius nothing."""
+    s = O.OracleScene(grid=(24, 20, 28))
+    s.set_hand(pos=(10.5, 8.5, 12.5), vel=(1.0, -2.0, 0.5), radius=2.0, strength=0.5,
+               active=True, eye=(10.5, 8.5, -40.0))
+    h = s.hand
+    assert list(h.bbox_min) == [8, 6, 0] and list(h.bbox_max) == [13, 11, 27]
+    cv = np.zeros(3 * s.G, s.L.real)
+    s.L.lib.orc_apply_hand(C.byref(s.params), C.byref(h), O._ptr(cv))
+    v = cv.reshape(28, 20, 24, 3)  # z, y, x
+    np.testing.assert_allclose(v[12, 8, 10], [0.5, -1.0, 0.25], rtol=0, atol=0)
+    np.testing.assert_allclose(v[3, 8, 11], [0.125, -0.25, 0.0625], rtol=0, atol=0)
+    np.testing.assert_allclose(v[27, 9, 10], [0.125, -0.25, 0.0625], rtol=0, atol=0)
+    assert not v[:, 8, 12].any() and not v[:, 6, 10].any() and not v[:, :, 5:7].any()
+
+
+def test_append_extends_with_unit_mass_and_zero_affine():
+    s = O.OracleScene(grid=(16, 16, 16))
+    s.reset(np.array([[5, 5, 5]], np.float32), C9=np.ones((1, 9), np.float32))
+    s.append(np.array([[6, 6, 6], [7, 7, 7]], np.float32), np.array([[1, 2, 3], [4, 5, 6]]))
+    assert s.n == 3 and s.params.particle_num == 3
+    assert (s.mass == 1).all() and (s.affine[1:] == 0).all() and (s.affine[0] == 1).all()
+    np.testing.assert_array_equal(s.vel4[2, :3], [4, 5, 6])
