@@ -26,7 +26,8 @@ Prints one JSON line:
   cpu_baseline     the CPU restatement of the reference step on this host's
                    cores, on a bounded sample of the same workload
 Extra keys: per-stage mean ms, whole-step roofline fraction, p50/p99 step
-latency (from the event-timed region).
+latency (from the event-timed region), the renderer handoff (device export
+kernel against HBM, pinned frames alone and overlapped with the steps).
 """
 from __future__ import annotations
 
@@ -268,6 +269,59 @@ def serve_bench(args) -> dict:
     }
 
 
+def handoff_bench(s, n: int, steps_ms: float) -> dict:
+    """Renderer handoff at the workload's size: the device export kernel
+    (HIP events on the scene's stream around back-to-back exports) and the
+    pinned frames (export + device-to-host copy), alone and every 8 steps
+    while the steps run."""
+    from mlsmpm import capi
+    from mlsmpm.hipmem import DeviceBuffer, StreamTimer
+
+    bufs = [DeviceBuffer(n * 16), DeviceBuffer(n * 16), DeviceBuffer(n * 4)]
+    tm = StreamTimer(s.stream())
+    out = {}
+    for name, reset in (("export_storage_order", False), ("export_reset_order", True)):
+        s.export_device(*(b.ptr for b in bufs), reset_order=reset)
+        K = 50
+        tm.start()
+        for _ in range(K):
+            s.export_device(*(b.ptr for b in bufs), reset_order=reset)
+        ms = tm.stop() / K
+        # 28 B read (pos, vel, density; + 4 B id in reset order), 36 B written
+        alg = (64 + (4 if reset else 0)) * n
+        out[name] = {"ms": ms, "alg_bytes": alg, "GB/s": alg / (ms * 1e-3) / 1e9,
+                     "frac": alg / (ms * 1e-3) / HBM_PEAK}
+    tm.close()
+    for b in bufs:
+        b.close()
+    what = capi.MPM_EXPORT_POS | capi.MPM_EXPORT_VEL | capi.MPM_EXPORT_DENSITY
+    s.frame_poll(s.frame_begin(what), wait=True, copy=False)
+    K = 10
+    t0 = time.perf_counter()
+    for _ in range(K):
+        s.frame_poll(s.frame_begin(what), wait=True, copy=False)
+    frame_ms = (time.perf_counter() - t0) / K * 1e3
+    out["pinned_frame_ms"] = frame_ms
+    out["pinned_GB/s"] = 36 * n / (frame_ms * 1e-3) / 1e9
+    # a frame every 8 steps: begin, step, then the previous frame is waited for
+    s.sync()
+    F, rounds = 8, 25
+    t0 = time.perf_counter()
+    prev = None
+    for _ in range(rounds):
+        t = s.frame_begin(what)
+        s.step(F)
+        if prev is not None:
+            s.frame_poll(prev, wait=True, copy=False)
+        prev = t
+    s.frame_poll(prev, wait=True, copy=False)
+    s.sync()
+    per_step = (time.perf_counter() - t0) / (F * rounds) * 1e3
+    out["steps_with_frame_every_8"] = {"ms_per_step": per_step,
+                                       "vs_steps_alone": per_step / steps_ms}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +350,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-chunk", type=int, default=16)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-handoff", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES each (the plain copy that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -440,6 +495,9 @@ def main() -> int:
             "p99": float(np.percentile(step_ms, 99)) if len(step_ms) else None,
         },
     }
+    if not args.no_handoff:
+        print("[bench] renderer handoff ...", file=sys.stderr, flush=True)
+        line["handoff"] = handoff_bench(s, n, step_ms_mean)
     s.close()
     if not args.no_batch:
         line["batched"] = batch_bench(args)

@@ -136,6 +136,66 @@ int mpm_read_grid(mpm_scene *s, int32_t *cell_mass, int32_t *cell_mv3, float *ce
  * the free-list stack and its top.  Any output may be NULL. */
 int mpm_read_diffuse(mpm_scene *s, void *rec48, uint32_t *free_list, int32_t *top);
 
+/* ---- Renderer handoff ------------------------------------------------------
+ * The reference's renderer binds the simulation buffers directly, without a
+ * read-back: particlePosBuffer and particleVelBuffer as stride-16 instance
+ * vertex buffers (fluid-renderer.js:315-316), particleDensityBuffer as
+ * storage (fluid-renderer.js:232, 256) and DiffuseGPU.particleBuffer as a
+ * stride-48 vertex buffer over all maxCount slots, dead slots culled by
+ * their alive flag (fluid-renderer.js:850-856).  Two handoffs replace that
+ * binding: a device-to-device export into buffers the caller owns (a
+ * renderer on the same GPU), and a double-buffered export into pinned host
+ * memory whose copy overlaps the following steps (a renderer elsewhere; the
+ * mapAsync pattern of pollDelete / pollDebug). */
+#define MPM_EXPORT_POS 1u           /* n float4 (x, y, z, 0) */
+#define MPM_EXPORT_VEL 2u           /* n float4 (vx, vy, vz, 0) */
+#define MPM_EXPORT_DENSITY 4u       /* n floats */
+#define MPM_EXPORT_DIFFUSE 8u       /* diffuse_capacity 48-byte records */
+#define MPM_EXPORT_ALL 15u
+/* Particles in reset order (as mpm_read_particles).  Without it they come in
+ * the scene's current storage order: the same particles, pos, vel and
+ * density row for row, read and written contiguously (what a renderer
+ * drawing instances needs). */
+#define MPM_EXPORT_RESET_ORDER 16u
+
+typedef struct mpm_export {
+  float *pos4;     /* device pointers, NULL to skip */
+  float *vel4;
+  float *density;
+  void *diffuse48;
+} mpm_export;
+
+/* Enqueue on the scene's stream (mpm_get_stream) the renderer layouts of the
+ * current state into the caller's device buffers: n = active particles at
+ * enqueue time (mpm_get_counts).  flags: 0 or MPM_EXPORT_RESET_ORDER.
+ * Returns at once; order later work after it with the scene's stream or
+ * mpm_sync. */
+int mpm_export_device(mpm_scene *s, const mpm_export *dst, uint32_t flags);
+
+/* One exported frame in pinned host memory. */
+typedef struct mpm_frame {
+  const float *pos4;        /* NULL unless asked for */
+  const float *vel4;
+  const float *density;
+  const void *diffuse48;
+  uint32_t n;               /* particles in the frame */
+  int32_t top;              /* diffuse free-list top (alive = capacity - top); 0 without a pool */
+  uint64_t steps;           /* steps since reset when the frame was taken */
+  uint64_t ticket;
+} mpm_frame;
+
+/* Take a frame of the current state (what: MPM_EXPORT_* bits, plus
+ * MPM_EXPORT_RESET_ORDER): the layouts are written to device staging on the
+ * scene's stream and copied to pinned host memory on a copy stream, so the
+ * steps enqueued after this call run while the copy does.  Two frames are
+ * kept: a frame's memory is reused by the second mpm_frame_begin after it. */
+int mpm_frame_begin(mpm_scene *s, uint32_t what, uint64_t *ticket);
+
+/* *ready = 1 and *frame filled once the frame's copy has finished (wait != 0
+ * blocks until then); *ready = 0 otherwise.  MPM_ERR_NOT_FOUND when the
+ * ticket's memory has been reused.  The pointers stay valid until then. */
+int mpm_frame_poll(mpm_scene *s, uint64_t ticket, int wait, int *ready, mpm_frame *frame);
+
 /* Overwrite the diffuse pool, free list and top (checkpoint restore, tests). */
 int mpm_write_diffuse(mpm_scene *s, const void *rec48, const uint32_t *free_list, int32_t top);
 

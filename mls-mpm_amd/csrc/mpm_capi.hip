@@ -237,6 +237,21 @@ int mpm_append_particles(mpm_scene *s, uint32_t n, const float *pos3, const floa
   return scene_append(s, n, pos3, vel3);
 }
 
+int mpm_export_device(mpm_scene *s, const mpm_export *dst, uint32_t flags) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_export_device: scene is NULL");
+  return scene_export_device(s, dst, flags);
+}
+
+int mpm_frame_begin(mpm_scene *s, uint32_t what, uint64_t *ticket) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_frame_begin: scene is NULL");
+  return scene_frame_begin(s, what, ticket);
+}
+
+int mpm_frame_poll(mpm_scene *s, uint64_t ticket, int wait, int *ready, mpm_frame *frame) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_frame_poll: scene is NULL");
+  return scene_frame_poll(s, ticket, wait, ready, frame);
+}
+
 int mpm_get_stream(mpm_scene *s, void **st) {
   if (!s || !st) return fail(MPM_ERR_INVALID, "mpm_get_stream: NULL argument");
   *st = (void *)s->stream;

@@ -61,7 +61,10 @@ uint32_t scan_scratch_words(uint32_t n);
 void launch_scan(const uint32_t *in, uint32_t n, bool popc, uint32_t *out, uint32_t *scratch,
                  uint32_t *total, hipStream_t st, uint32_t *clear = nullptr);
 void launch_export_particles(const PState &S, uint32_t n, float4 *pos4, float4 *vel4, float *C9,
-                             float *dens, hipStream_t st);
+                             float *dens, hipStream_t st, bool reset_order = true);
+// Copies bytes (a multiple of 16) with a kernel: works on any device pointer
+// of this process, whichever runtime allocated it.
+void launch_copy16(const void *src, void *dst, size_t bytes, hipStream_t st);
 void launch_export_grid(const GState &g, uint32_t G, int32_t *m, int32_t *mv3, float *mf,
                         float *v3, hipStream_t st);
 // Slots first .. first+n-1 (ids alike) from host-layout arrays.

@@ -496,11 +496,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in, 
 
 // ---------------------------------------------------------------------------
 // Export to the reference layouts (reset order), for reads.
+// The renderer layouts of the particles: into reset-order rows (RESET) or
+// into the storage order (contiguous reads and writes).
+template <bool RESET>
 __global__ __launch_bounds__(256) void k_export_particles(PState S, uint32_t n, float4 *pos4,
                                                           float4 *vel4, float *C9, float *dens) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  uint32_t id = S.id[s];
+  const uint32_t id = RESET ? S.id[s] : s;
   if (pos4) pos4[id] = make_float4(S.at(PX, s), S.at(PY, s), S.at(PZ, s), 0.0f);
   if (vel4) vel4[id] = make_float4(S.at(VX, s), S.at(VY, s), S.at(VZ, s), 0.0f);
   if (C9) {
@@ -621,8 +624,23 @@ void launch_diffuse(const Params &P, const PState &S, const GState &g, const Wal
 }
 
 void launch_export_particles(const PState &S, uint32_t n, float4 *pos4, float4 *vel4, float *C9,
-                             float *dens, hipStream_t st) {
-  if (n) k_export_particles<<<blocks(n, 256), 256, 0, st>>>(S, n, pos4, vel4, C9, dens);
+                             float *dens, hipStream_t st, bool reset_order) {
+  if (!n) return;
+  if (reset_order) k_export_particles<true><<<blocks(n, 256), 256, 0, st>>>(S, n, pos4, vel4, C9, dens);
+  else k_export_particles<false><<<blocks(n, 256), 256, 0, st>>>(S, n, pos4, vel4, C9, dens);
+}
+
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *src, uint4 *dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy16(const void *src, void *dst, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / 16;
+  if (!n) return;
+  const size_t want = (n + 255) / 256;
+  k_copy16<<<(uint32_t)(want < 4096 ? want : 4096), 256, 0, st>>>((const uint4 *)src, (uint4 *)dst, n);
 }
 void launch_export_grid(const GState &g, uint32_t G, int32_t *m, int32_t *mv3, float *mf,
                         float *v3, hipStream_t st) {

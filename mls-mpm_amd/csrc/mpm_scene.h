@@ -79,6 +79,22 @@ struct mpm_scene {
   bool hand_on = false;
   uint32_t *ibits = nullptr, *iprefix = nullptr, *iscratch = nullptr, *itotal = nullptr,
            *ihit = nullptr;
+  // Renderer handoff: two frames, each written to device staging on the
+  // scene's stream and copied into pinned host memory on copy_stream.
+  struct Frame {
+    char *dev = nullptr;          // device staging (freed with owned[])
+    char *host = nullptr;         // pinned, frame_bytes
+    hipEvent_t ready = nullptr;   // staging written
+    hipEvent_t done = nullptr;    // host copy finished
+    uint64_t ticket = 0;          // 0: never used
+    uint64_t steps = 0;
+    uint32_t n = 0, what = 0;
+    size_t off[5] = {};           // pos, vel, density, diffuse records, top
+  };
+  Frame frames[2];
+  size_t frame_bytes = 0;
+  hipStream_t copy_stream = nullptr;
+  uint64_t frame_next = 1;
   // Batch membership: a member's work runs on the batch's stream.
   mpm_batch *batch = nullptr;
   hipStream_t own_stream = nullptr;
@@ -111,6 +127,9 @@ int scene_delete_near(mpm_scene *s, const float *origin, const float *dir, float
 int scene_raycast(mpm_scene *s, const float *origin, const float *dir, float radius, float *t,
                   int *hit);
 int scene_append(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3);
+int scene_export_device(mpm_scene *s, const mpm_export *dst, uint32_t flags);
+int scene_frame_begin(mpm_scene *s, uint32_t what, uint64_t *ticket);
+int scene_frame_poll(mpm_scene *s, uint64_t ticket, int wait, int *ready, mpm_frame *frame);
 int scene_enable_timing(mpm_scene *s, bool on);
 int scene_stage_times(mpm_scene *s, float *ms8);
 int scene_tile_stats(mpm_scene *s, uint64_t out[4]);

@@ -364,6 +364,15 @@ void scene_free(mpm_scene *s) {
     (void)hipSetDevice(s->cfg.device);
     (void)hipStreamSynchronize(s->stream);
   }
+  if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
+  for (mpm_scene::Frame &f : s->frames) {
+    if (f.host) (void)hipHostFree(f.host);
+    if (f.ready) (void)hipEventDestroy(f.ready);
+    if (f.done) (void)hipEventDestroy(f.done);
+    f = mpm_scene::Frame{};
+  }
+  if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+  s->copy_stream = nullptr;
   for (int i = 0; i < s->nowned; i++) (void)hipFree(s->owned[i]);
   s->nowned = 0;
   if (s->stage) (void)hipFree(s->stage);
@@ -1085,6 +1094,133 @@ int scene_append(mpm_scene *s, uint32_t k, const float *pos3, const float *vel3)
   s->since_sort = 0;
   s->sorted_once = false;
   s->version++;
+  return MPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Renderer handoff (fluid-renderer.js:232, 256, 315-316, 850-856).
+
+int scene_export_device(mpm_scene *s, const mpm_export *dst, uint32_t flags) {
+  if (!dst) return failf(MPM_ERR_INVALID, "mpm_export_device: dst is NULL");
+  if (flags & ~MPM_EXPORT_RESET_ORDER)
+    return failf(MPM_ERR_INVALID, "mpm_export_device: flags 0x%x (only MPM_EXPORT_RESET_ORDER)",
+                 flags);
+  if (dst->diffuse48 && !s->cfg.diffuse_capacity)
+    return failf(MPM_ERR_STATE, "scene has no diffuse pool (diffuse_capacity = 0)");
+  HIPCHK(hipSetDevice(s->cfg.device));
+  launch_export_particles(pstate(s, s->cur), s->n, (float4 *)dst->pos4, (float4 *)dst->vel4,
+                          nullptr, dst->density, s->stream,
+                          (flags & MPM_EXPORT_RESET_ORDER) != 0);
+  if (dst->diffuse48)
+    launch_copy16(s->d.pool, dst->diffuse48, (size_t)s->cfg.diffuse_capacity * 48, s->stream);
+  HIPCHK(hipGetLastError());
+  return MPM_OK;
+}
+
+// The two frames, at first use: staging and pinned memory for every part at
+// capacity, so a frame never reallocates.
+static int ensure_frames(mpm_scene *s) {
+  if (s->copy_stream) return MPM_OK;
+  const size_t cap = s->cfg.capacity, D = s->cfg.diffuse_capacity;
+  s->frame_bytes = cap * 36 + D * 48 + 4 * 256;
+  // (a failure part way leaves what was made, for the next call to finish)
+  for (mpm_scene::Frame &f : s->frames) {
+    if (!f.dev) DALLOC(f.dev, s->frame_bytes);
+    if (!f.host) {
+      hipError_t e = hipHostMalloc((void **)&f.host, s->frame_bytes, hipHostMallocDefault);
+      if (e != hipSuccess) {
+        f.host = nullptr;
+        (void)hipGetLastError();
+        return failf(MPM_ERR_NOMEM, "pinned hipHostMalloc(%zu) failed: %s", s->frame_bytes,
+                     hipGetErrorString(e));
+      }
+    }
+    if (!f.ready) HIPCHK(hipEventCreateWithFlags(&f.ready, hipEventDisableTiming));
+    if (!f.done) HIPCHK(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
+  }
+  HIPCHK(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+  return MPM_OK;
+}
+
+int scene_frame_begin(mpm_scene *s, uint32_t what, uint64_t *ticket) {
+  if (!ticket) return failf(MPM_ERR_INVALID, "mpm_frame_begin: ticket is NULL");
+  if (what & ~(MPM_EXPORT_ALL | MPM_EXPORT_RESET_ORDER) || !(what & MPM_EXPORT_ALL))
+    return failf(MPM_ERR_INVALID, "mpm_frame_begin: what 0x%x", what);
+  const size_t D = s->cfg.diffuse_capacity;
+  if ((what & MPM_EXPORT_DIFFUSE) && !D)
+    return failf(MPM_ERR_STATE, "scene has no diffuse pool (diffuse_capacity = 0)");
+  HIPCHK(hipSetDevice(s->cfg.device));
+  int rc = ensure_frames(s);
+  if (rc) return rc;
+  mpm_scene::Frame &f = s->frames[s->frame_next & 1];
+  // this frame's staging and host memory are free once its last copy is
+  if (f.ticket) HIPCHK(hipStreamWaitEvent(s->stream, f.done, 0));
+  const size_t n = s->n;
+  // the parts asked for, packed, so that one copy moves the frame
+  size_t o = 0;
+  auto part = [&](int k, uint32_t bit, size_t bytes) {
+    f.off[k] = o;
+    if (what & bit) o = (o + bytes + 255) & ~(size_t)255;
+  };
+  part(0, MPM_EXPORT_POS, n * 16);
+  part(1, MPM_EXPORT_VEL, n * 16);
+  part(2, MPM_EXPORT_DENSITY, n * 4);
+  part(3, MPM_EXPORT_DIFFUSE, D * 48);
+  part(4, MPM_EXPORT_DIFFUSE, 4);
+  launch_export_particles(pstate(s, s->cur), (uint32_t)n,
+                          (what & MPM_EXPORT_POS) ? (float4 *)(f.dev + f.off[0]) : nullptr,
+                          (what & MPM_EXPORT_VEL) ? (float4 *)(f.dev + f.off[1]) : nullptr,
+                          nullptr,
+                          (what & MPM_EXPORT_DENSITY) ? (float *)(f.dev + f.off[2]) : nullptr,
+                          s->stream, (what & MPM_EXPORT_RESET_ORDER) != 0);
+  if (what & MPM_EXPORT_DIFFUSE) {
+    launch_copy16(s->d.pool, f.dev + f.off[3], D * 48, s->stream);
+    HIPCHK(hipMemcpyAsync(f.dev + f.off[4], s->d.top, 4, hipMemcpyDeviceToDevice, s->stream));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(f.ready, s->stream));
+  HIPCHK(hipStreamWaitEvent(s->copy_stream, f.ready, 0));
+  if (o) HIPCHK(hipMemcpyAsync(f.host, f.dev, o, hipMemcpyDeviceToHost, s->copy_stream));
+  HIPCHK(hipEventRecord(f.done, s->copy_stream));
+  f.ticket = s->frame_next++;
+  f.steps = s->steps;
+  f.n = (uint32_t)n;
+  f.what = what;
+  *ticket = f.ticket;
+  return MPM_OK;
+}
+
+int scene_frame_poll(mpm_scene *s, uint64_t ticket, int wait, int *ready, mpm_frame *out) {
+  if (!ready || !out) return failf(MPM_ERR_INVALID, "mpm_frame_poll: NULL argument");
+  *ready = 0;
+  mpm_scene::Frame *f = nullptr;
+  for (mpm_scene::Frame &x : s->frames)
+    if (ticket && x.ticket == ticket) f = &x;
+  if (!f)
+    return failf(MPM_ERR_NOT_FOUND, "frame %llu is unknown or its memory was reused",
+                 (unsigned long long)ticket);
+  HIPCHK(hipSetDevice(s->cfg.device));
+  if (wait) {
+    HIPCHK(hipEventSynchronize(f->done));
+  } else {
+    hipError_t e = hipEventQuery(f->done);
+    if (e == hipErrorNotReady) {
+      (void)hipGetLastError();
+      return MPM_OK;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipEventQuery(frame)");
+  }
+  const uint32_t w = f->what;
+  out->pos4 = (w & MPM_EXPORT_POS) ? (const float *)(f->host + f->off[0]) : nullptr;
+  out->vel4 = (w & MPM_EXPORT_VEL) ? (const float *)(f->host + f->off[1]) : nullptr;
+  out->density = (w & MPM_EXPORT_DENSITY) ? (const float *)(f->host + f->off[2]) : nullptr;
+  out->diffuse48 = (w & MPM_EXPORT_DIFFUSE) ? (const void *)(f->host + f->off[3]) : nullptr;
+  out->top = 0;
+  if (w & MPM_EXPORT_DIFFUSE) std::memcpy(&out->top, f->host + f->off[4], 4);
+  out->n = f->n;
+  out->steps = f->steps;
+  out->ticket = f->ticket;
+  *ready = 1;
   return MPM_OK;
 }
 

@@ -497,6 +497,71 @@ static napi_value js_append(napi_env env, napi_callback_info info) {
   return v;
 }
 
+/* frameBegin(handle, what) -> ticket (renderer handoff through pinned memory;
+ * the bindings of fluid-renderer.js:315-316, 850-856) */
+static napi_value js_frame_begin(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  mpm_scene *s = scene_of(env, a[0]);
+  if (!s) return NULL;
+  uint64_t t = 0;
+  MPM_CALL(env, mpm_frame_begin(s, u32_of(env, a[1]), &t));
+  napi_value v;
+  napi_create_double(env, (double)t, &v);
+  return v;
+}
+
+/* framePoll(handle, ticket, wait, pos4?, vel4?, density?, diffuse?) ->
+ * null while the copy runs, else {n, top, steps}; the parts the frame holds
+ * are copied into the outputs given (Float32Array n*4 / n*4 / n, ArrayBuffer
+ * or TypedArray of 48 * maxCount bytes). */
+static napi_value js_frame_poll(napi_env env, napi_callback_info info) {
+  napi_value a[7];
+  if (!get_args(env, info, 7, a)) return NULL;
+  mpm_scene *s = scene_of(env, a[0]);
+  if (!s) return NULL;
+  const uint64_t ticket = (uint64_t)f64_of(env, a[1]);
+  bool wait = false;
+  napi_coerce_to_bool(env, a[2], &a[2]);
+  napi_get_value_bool(env, a[2], &wait);
+  int ready = 0;
+  mpm_frame f;
+  MPM_CALL(env, mpm_frame_poll(s, ticket, wait ? 1 : 0, &ready, &f));
+  napi_value v;
+  if (!ready) {
+    napi_get_null(env, &v);
+    return v;
+  }
+  const void *src[4] = {f.pos4, f.vel4, f.density, f.diffuse48};
+  mpm_config c;
+  MPM_CALL(env, mpm_get_config(s, &c));
+  const size_t need[4] = {(size_t)f.n * 16, (size_t)f.n * 16, (size_t)f.n * 4,
+                          (size_t)c.diffuse_capacity * 48};
+  for (int i = 0; i < 4; i++) {
+    void *dst;
+    size_t bytes;
+    if (!bytes_of(env, a[3 + i], &dst, &bytes)) {
+      napi_throw_type_error(env, NULL, "framePoll: outputs must be TypedArray / ArrayBuffer or null");
+      return NULL;
+    }
+    if (!dst || !src[i]) continue;
+    if (bytes < need[i]) {
+      napi_throw_range_error(env, NULL, "framePoll: output too short for the frame");
+      return NULL;
+    }
+    memcpy(dst, src[i], need[i]);
+  }
+  napi_create_object(env, &v);
+  napi_value x;
+  napi_create_uint32(env, f.n, &x);
+  napi_set_named_property(env, v, "n", x);
+  napi_create_int32(env, f.top, &x);
+  napi_set_named_property(env, v, "top", x);
+  napi_create_double(env, (double)f.steps, &x);
+  napi_set_named_property(env, v, "steps", x);
+  return v;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
   (void)info;
   napi_value v;
@@ -525,6 +590,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"deleteNear", js_delete_near},
       {"raycast", js_raycast},
       {"append", js_append},
+      {"frameBegin", js_frame_begin},
+      {"framePoll", js_frame_poll},
   };
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;

@@ -26,6 +26,8 @@ const require = createRequire(import.meta.url);
 const addon = require('./addon/mlsmpm.node');
 
 export const MAX_WALLS = 4;
+// Parts of a frame (mpm_frame_begin): bit flags.
+export const FRAME = { POS: 1, VEL: 2, DENSITY: 4, DIFFUSE: 8, ALL: 15, RESET_ORDER: 16 };
 
 export class FluidHIP {
     constructor(aspectX, aspectY, aspectZ, particleRadius, particleNum) {
@@ -397,6 +399,27 @@ export class FluidHIP {
     }
 
     pollRaycast() {}
+
+    // -- renderer handoff (fluid-renderer.js:232, 256, 315-316, 850-856) ---
+
+    // Start copying the renderer's buffers (FRAME bits) into pinned memory;
+    // the copy overlaps the simFrame() calls that follow.
+    beginFrame(what = FRAME.ALL) {
+        if (!this.diffuse) what &= ~FRAME.DIFFUSE;
+        this._frameTicket = addon.frameBegin(this._scene, what);
+        return this._frameTicket;
+    }
+
+    // The frame begun last, copied into `out` ({pos4, vel4, density,
+    // diffuse}: arrays sized for the frame, any may be missing), or null
+    // while its copy still runs (wait = true blocks instead).
+    pollFrame(out = {}, wait = false) {
+        if (!this._frameTicket) return null;
+        const r = addon.framePoll(this._scene, this._frameTicket, wait, out.pos4 || null,
+                                  out.vel4 || null, out.density || null, out.diffuse || null);
+        if (r) Object.assign(r, out);
+        return r;
+    }
 
     stepsTaken() {
         return this._scene ? addon.counts(this._scene).steps : 0;

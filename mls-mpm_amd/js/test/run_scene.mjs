@@ -4,7 +4,7 @@
 // for comparison with the CPU restatement.
 import fs from 'fs';
 import path from 'path';
-import { FluidHIP } from '../fluid-hip.mjs';
+import { FluidHIP, FRAME } from '../fluid-hip.mjs';
 import { DiffuseHIP } from '../diffuse-hip.mjs';
 
 const dir = process.argv[2];
@@ -38,6 +38,15 @@ walls.forEach((w, i) => wall.set(w, i * w.length));
 put('walls.f32', wall);
 const pool = new Uint8Array(d.readParticles());
 put('pool.bin', pool);
+// the same state through the pinned frame handoff
+f.beginFrame(FRAME.ALL | FRAME.RESET_ORDER);
+const out = { pos4: new Float32Array(4 * n), vel4: new Float32Array(4 * n),
+              density: new Float32Array(n), diffuse: new ArrayBuffer(8192 * 48) };
+const fr = f.pollFrame(out, true);
+put('frame_pos4.f32', out.pos4);
+put('frame_vel4.f32', out.vel4);
+put('frame_pool.bin', new Uint8Array(out.diffuse));
 f.sync();
-console.log(JSON.stringify({ n, frames, alive: d.aliveCount, grid: [f.grid_X_num, f.grid_Y_num, f.grid_Z_num] }));
+console.log(JSON.stringify({ n, frames, alive: d.aliveCount, grid: [f.grid_X_num, f.grid_Y_num, f.grid_Z_num],
+                             frame: { n: fr.n, top: fr.top, steps: fr.steps } }));
 f.destroy();

@@ -21,6 +21,13 @@ MPM_ERR_UNSUPPORTED = -6
 MPM_ERR_STATE = -7
 MPM_ERR_NOT_FOUND = -8
 
+MPM_EXPORT_POS = 1
+MPM_EXPORT_VEL = 2
+MPM_EXPORT_DENSITY = 4
+MPM_EXPORT_DIFFUSE = 8
+MPM_EXPORT_ALL = 15
+MPM_EXPORT_RESET_ORDER = 16
+
 MPM_FLAG_KEEP_GRID = 1
 MPM_FLAG_NO_LDS = 2
 MPM_FLAG_NO_GRAPH = 4
@@ -114,6 +121,19 @@ class Hand(C.Structure):
 assert C.sizeof(Hand) == 48
 
 
+class Export(C.Structure):
+    """mpm_export: device pointers of the renderer buffers (0 = skip)."""
+    _fields_ = [("pos4", C.c_void_p), ("vel4", C.c_void_p), ("density", C.c_void_p),
+                ("diffuse48", C.c_void_p)]
+
+
+class Frame(C.Structure):
+    """mpm_frame: one exported frame in pinned host memory."""
+    _fields_ = [("pos4", C.c_void_p), ("vel4", C.c_void_p), ("density", C.c_void_p),
+                ("diffuse48", C.c_void_p), ("n", C.c_uint32), ("top", C.c_int32),
+                ("steps", C.c_uint64), ("ticket", C.c_uint64)]
+
+
 # Every entry point declared in include/mlsmpm.h, with its argument types.
 _VP = C.c_void_p
 _SIGNATURES = {
@@ -140,6 +160,9 @@ _SIGNATURES = {
     "mpm_delete_near": ([_VP, _VP, _VP, C.c_float, _VP], C.c_int),
     "mpm_raycast": ([_VP, _VP, _VP, C.c_float, _VP, _VP], C.c_int),
     "mpm_append_particles": ([_VP, C.c_uint32, _VP, _VP], C.c_int),
+    "mpm_export_device": ([_VP, _VP, C.c_uint32], C.c_int),
+    "mpm_frame_begin": ([_VP, C.c_uint32, _VP], C.c_int),
+    "mpm_frame_poll": ([_VP, C.c_uint64, C.c_int, _VP, _VP], C.c_int),
     "mpm_get_stream": ([_VP, _VP], C.c_int),
     "mpm_enable_timing": ([_VP, C.c_int], C.c_int),
     "mpm_stage_times": ([_VP, _VP], C.c_int),

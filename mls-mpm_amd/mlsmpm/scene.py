@@ -198,6 +198,50 @@ class Scene:
         vel = _f32(vel, (k, 3))
         check(load().mpm_append_particles(self._h, k, _p(pos), _p(vel)))
 
+    # -- renderer handoff (fluid-renderer.js:232, 256, 315-316, 850-856) ----
+    def export_device(self, pos4=0, vel4=0, density=0, diffuse48=0, reset_order=False):
+        """Enqueue the renderer layouts into device buffers (integer device
+        addresses, 0 = skip) on the scene's stream; returns at once."""
+        e = capi.Export(pos4 or None, vel4 or None, density or None, diffuse48 or None)
+        check(load().mpm_export_device(self._h, C.byref(e),
+                                       capi.MPM_EXPORT_RESET_ORDER if reset_order else 0))
+
+    def frame_begin(self, what: int = capi.MPM_EXPORT_ALL, reset_order=False) -> int:
+        """Take a frame into pinned host memory; the copy overlaps later steps."""
+        t = C.c_uint64()
+        if reset_order:
+            what |= capi.MPM_EXPORT_RESET_ORDER
+        check(load().mpm_frame_begin(self._h, what, C.byref(t)))
+        return t.value
+
+    def frame_poll(self, ticket: int, wait: bool = True, copy: bool = True):
+        """None while the frame's copy runs, else a dict: pos4 / vel4 (n, 4),
+        density (n,), diffuse (records), n, top, steps.  copy=False gives
+        views of the pinned memory, valid until the second frame_begin after
+        this frame's."""
+        ready = C.c_int()
+        f = capi.Frame()
+        check(load().mpm_frame_poll(self._h, ticket, 1 if wait else 0, C.byref(ready),
+                                    C.byref(f)))
+        if not ready.value:
+            return None
+        n = f.n
+
+        def view(ptr, count, dtype, shape):
+            if not ptr:
+                return None
+            a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_uint8)),
+                                      (count * np.dtype(dtype).itemsize,))
+            a = a.view(dtype).reshape(shape)
+            return a.copy() if copy else a
+
+        D = self.cfg.diffuse_capacity
+        return {"pos4": view(f.pos4, 4 * n, np.float32, (n, 4)),
+                "vel4": view(f.vel4, 4 * n, np.float32, (n, 4)),
+                "density": view(f.density, n, np.float32, (n,)),
+                "diffuse": view(f.diffuse48, D, DREC_DTYPE, (D,)),
+                "n": n, "top": f.top, "steps": f.steps, "ticket": f.ticket}
+
     def enable_timing(self, on: bool = True):
         check(load().mpm_enable_timing(self._h, 1 if on else 0))
 

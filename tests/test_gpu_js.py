@@ -45,6 +45,13 @@ def test_fluidhip_scene_matches_oracle(tmp_path):
     for f in ("pos", "vel", "ptype", "alive", "lifetime", "pad"):
         np.testing.assert_array_equal(pool[f], o.pool[f], err_msg=f"pool {f}")
     assert info["alive"] == 8192 - int(o.top[0])
+    # the pinned frame handoff holds the same final state
+    assert info["frame"] == {"n": n, "top": int(o.top[0]), "steps": frames}
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "frame_pos4.f32", np.float32).reshape(n, 4),
+                                  o.pos4)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "frame_vel4.f32", np.float32).reshape(n, 4),
+                                  o.vel4)
+    assert (tmp_path / "frame_pool.bin").read_bytes() == (tmp_path / "pool.bin").read_bytes()
 
 
 def test_fluidhip_interaction_matches_oracle(tmp_path):
