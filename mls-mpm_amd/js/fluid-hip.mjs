@@ -195,7 +195,8 @@ export class FluidHIP {
     _config() {
         const c = Object.assign({}, this._cfg);
         c.capacity = this.particle_num;
-        c.dt = this.DT;
+        // the substep's dt, as simFrame packs it (fluid-gpu.js:649-651)
+        c.dt = Math.fround(this.DT / this.SUBSTEPS);
         c.gravity = this.GRAVITY;
         c.rest_density = this.REST_DENSITY;
         c.stiffness = this.STIFFNESS;
@@ -245,11 +246,32 @@ export class FluidHIP {
         if (state) {
             addon.reset(this._scene, state.pos, state.vel, state.C, this.particle_mass.subarray(0, n));
         } else {
-            addon.reset(this._scene, this.particle_pos.subarray(0, 3 * n),
+            addon.reset(this._scene, this._inGrid(this.particle_pos.subarray(0, 3 * n)),
                         this.particle_vel.subarray(0, 3 * n),
                         this.particle_affine.subarray(0, 9 * n), this.particle_mass.subarray(0, n));
         }
         if (hand) this.updateHand(...hand);
+    }
+
+    // The scene takes particles whose 3x3x3 stencil lies inside the grid:
+    // coordinates in [1, grid - 1).  fillBlock's last lattice column can lie
+    // beyond grid - 1 (the mobile preset: 0.97 * 29 cells); the reference's
+    // first P2G then writes that column's outer stencil cells into the next
+    // row of its linear grid, and its G2P clamps the particles to the hard
+    // bounds.  Here such coordinates move to the largest float below
+    // grid - 1 before the upload.
+    _inGrid(pos) {
+        const lim = [this.grid_X_num - 1, this.grid_Y_num - 1, this.grid_Z_num - 1];
+        const f = new Float32Array(1), u = new Uint32Array(f.buffer);
+        const hi = lim.map((l) => { f[0] = l; u[0] -= 1; return f[0]; });
+        let out = pos;
+        for (let i = 0; i < pos.length; i++) {
+            const a = i % 3;
+            if (pos[i] >= 1 && pos[i] <= hi[a]) continue;
+            if (out === pos) out = Float32Array.from(pos);
+            out[i] = Math.min(hi[a], Math.max(1, pos[i]));
+        }
+        return out;
     }
 
     // A diffuse pool attached after initGPU: the scene is rebuilt with it,
@@ -279,7 +301,8 @@ export class FluidHIP {
         this._sent = c;
     }
 
-    // One frame: SUBSTEPS whole steps (P2G, grid update, G2P, diffuse).
+    // One frame: SUBSTEPS whole steps of DT / SUBSTEPS each (P2G, grid
+    // update, hand, G2P, diffuse).
     simFrame() {
         try {
             if (!this._scene) this.initGPU(this.device === null ? 0 : this.device);

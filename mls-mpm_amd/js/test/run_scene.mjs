@@ -9,6 +9,7 @@ import { DiffuseHIP } from '../diffuse-hip.mjs';
 
 const dir = process.argv[2];
 const frames = parseInt(process.argv[3] || '6', 10);
+const substeps = parseInt(process.argv[4] || '1', 10);
 let s = 12345;
 const rnd = () => {  // seeded jitter source (LCG, 2^32)
     s = (Math.imul(s, 1664525) + 1013904223) >>> 0;
@@ -19,6 +20,7 @@ f.random = rnd;
 const n = f.fillBlock(0.1, 0.1, 0.1, 0.6, 0.8, 0.5);
 const put = (name, a) => fs.writeFileSync(path.join(dir, name), Buffer.from(a.buffer, a.byteOffset, a.byteLength));
 put('pos3.f32', f.particle_pos.subarray(0, 3 * n));
+f.SUBSTEPS = substeps;
 f.initGPU(0);
 const d = new DiffuseHIP(8192);
 d.initGPU(0, f);
@@ -48,5 +50,5 @@ put('frame_vel4.f32', out.vel4);
 put('frame_pool.bin', new Uint8Array(out.diffuse));
 f.sync();
 console.log(JSON.stringify({ n, frames, alive: d.aliveCount, grid: [f.grid_X_num, f.grid_Y_num, f.grid_Z_num],
-                             frame: { n: fr.n, top: fr.top, steps: fr.steps } }));
+                             substeps, frame: { n: fr.n, top: fr.top, steps: fr.steps } }));
 f.destroy();

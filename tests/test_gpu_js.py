@@ -21,10 +21,13 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not shutil.which("node"), reason="node is missing")]
 
 
-def test_fluidhip_scene_matches_oracle(tmp_path):
+@pytest.mark.parametrize("substeps", [1, 3])
+def test_fluidhip_scene_matches_oracle(tmp_path, substeps):
+    """SUBSTEPS steps of DT / SUBSTEPS per simFrame (fluid-gpu.js:648-684),
+    the wall uniform packed once per frame."""
     frames = 6
     r = subprocess.run(["node", os.path.join(JS, "test", "run_scene.mjs"), str(tmp_path),
-                        str(frames)], capture_output=True, text=True, timeout=120)
+                        str(frames), str(substeps)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     info = json.loads(r.stdout.strip().splitlines()[-1])
     n, grid = info["n"], tuple(info["grid"])
@@ -33,10 +36,11 @@ def test_fluidhip_scene_matches_oracle(tmp_path):
     walls = np.fromfile(tmp_path / "walls.f32", np.float32).reshape(frames, 84)
     assert walls[0, 0] == 1 and walls[-1, 4 + 16 + 1] != 0  # one wall, turning at the end
     o = oracle.OracleScene(grid=grid, diffuse_max=8192, seed=1)
+    o.params.dt = float(np.float32(0.3 / substeps))
     o.reset(pos3)
     for k in range(frames):
         o.walls = np.ascontiguousarray(walls[k])
-        o.step(1)
+        o.step(substeps)
         p4 = np.fromfile(tmp_path / f"pos4_{k}.f32", np.float32).reshape(n, 4)
         v4 = np.fromfile(tmp_path / f"vel4_{k}.f32", np.float32).reshape(n, 4)
         np.testing.assert_array_equal(p4, o.pos4, err_msg=f"pos frame {k}")
@@ -46,7 +50,7 @@ def test_fluidhip_scene_matches_oracle(tmp_path):
         np.testing.assert_array_equal(pool[f], o.pool[f], err_msg=f"pool {f}")
     assert info["alive"] == 8192 - int(o.top[0])
     # the pinned frame handoff holds the same final state
-    assert info["frame"] == {"n": n, "top": int(o.top[0]), "steps": frames}
+    assert info["frame"] == {"n": n, "top": int(o.top[0]), "steps": frames * substeps}
     np.testing.assert_array_equal(np.fromfile(tmp_path / "frame_pos4.f32", np.float32).reshape(n, 4),
                                   o.pos4)
     np.testing.assert_array_equal(np.fromfile(tmp_path / "frame_vel4.f32", np.float32).reshape(n, 4),

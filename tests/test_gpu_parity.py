@@ -413,3 +413,37 @@ def test_moving_walls_match_oracle(mode):
         o.step(1)
         check_equal(s, o, f"step {step}", diffuse=True)
     s.close()
+
+
+# The reference's two presets (main.js:19-21, config.js:1-7): aspect 2x2x3,
+# particle radius 0.0125 / 0.0175, capacity 400,000 / 80,000, diffuse pool
+# 30,000 / 12,000 slots, fillBlock(0.03, 0.03, 0.03, 0.97, 0.97, 0.37).
+# WG_SIZE 64 / 32 shapes only the WebGPU dispatches and has no counterpart.
+PRESETS = {"desktop": (0.0125, 400_000, 30_000), "mobile": (0.0175, 80_000, 12_000)}
+
+
+@pytest.mark.parametrize("preset", list(PRESETS))
+def test_reference_presets(preset):
+    radius, cap, D = PRESETS[preset]
+    cfg = mlsmpm.config_from_aspect(2, 2, 3, radius, cap)
+    cfg.flags = KEEP
+    cfg.diffuse_capacity = D
+    grid = (cfg.grid_x, cfg.grid_y, cfg.grid_z)
+    assert grid == {"desktop": (40, 40, 60), "mobile": (29, 29, 43)}[preset]
+    pos = scenes.fill_block(grid, 0.03, 0.03, 0.03, 0.97, 0.97, 0.37,
+                            np.random.default_rng(5), capacity=cap)
+    # the scene takes coordinates in [1, grid - 1): the mobile fill's last
+    # column (x, y near 28.12 of 29 cells) moves inside, as FluidHIP uploads it
+    hi = np.nextafter(np.float32(np.array(grid) - 1), np.float32(0))
+    moved = int(np.count_nonzero(pos > hi))
+    assert (moved > 0) == (preset == "mobile")
+    pos = np.minimum(pos, hi).astype(np.float32)
+    s = mlsmpm.Scene(cfg)
+    s.reset(pos)
+    o = oracle.OracleScene(grid=grid, diffuse_max=D, seed=cfg.seed)
+    o.reset(pos)
+    for step in range(6):
+        s.step(1)
+        o.step(1)
+        check_equal(s, o, f"{preset} step {step}", diffuse=True)
+    s.close()

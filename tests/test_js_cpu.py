@@ -54,3 +54,16 @@ def test_fluidhip_fails_loudly_without_device():
     d = run_surface()
     assert d["init"]["code"] == "-4" and "device" in d["init"]["message"]
     assert d["simFrame"] is False and d["lastError"] == "-4"
+
+
+def test_mobile_preset_upload_and_host_spawn():
+    """The mobile preset's fill (main.js:19-21) is uploaded inside
+    [1, grid - 1); spawnParticles before initGPU (fluid-gpu.js:145-180) fills
+    the host arrays inside the hard bounds with unit mass, up to capacity."""
+    m = run_surface()["mobile"]
+    assert m["grid"] == [29, 29, 43] and m["filled"] == 80000
+    assert m["moved"] > 0
+    assert all(hi < g - 1 for hi, g in zip(m["max"], m["grid"]))
+    assert all(lo >= 1 for lo in m["min"])
+    assert m["spawned"] == 400 and m["active"] == 400
+    assert m["spawnedInside"] and m["massOne"]
