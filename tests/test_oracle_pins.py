@@ -497,3 +497,36 @@ def test_append_extends_with_unit_mass_and_zero_affine():
     assert s.n == 3 and s.params.particle_num == 3
     assert (s.mass == 1).all() and (s.affine[1:] == 0).all() and (s.affine[0] == 1).all()
     np.testing.assert_array_equal(s.vel4[2, :3], [4, 5, 6])
+
+
+def test_hand_push_oblique_ray_matches_the_geometry():
+    """An eye ray along (0.6, 0.8, 0): every cell's push equals
+    vel * (1 - d/r)^2 * strength with d the float64 distance of the cell
+    centre from the line (within f32 rounding), the push is zero exactly
+    where d >= r, and every pushed cell lies inside the computed cell box."""
+    g = (24, 20, 16)
+    s = O.OracleScene(grid=g)
+    pos, d = np.array([10.5, 8.5, 7.25]), np.array([0.6, 0.8, 0.0])
+    r, k, vel = 3.0, 0.75, np.array([1.0, -0.5, 2.0])
+    s.set_hand(pos=pos, vel=vel, radius=r, strength=k, active=True, eye=pos - 10.0 * d)
+    h = s.hand
+    cv = np.zeros(3 * s.G, s.L.real)
+    s.L.lib.orc_apply_hand(C.byref(s.params), C.byref(h), O._ptr(cv))
+    v = cv.reshape(g[2], g[1], g[0], 3)
+    z, y, x = np.meshgrid(np.arange(g[2]), np.arange(g[1]), np.arange(g[0]), indexing="ij")
+    to = np.stack([x + 0.5, y + 0.5, z + 0.5], -1) - pos
+    perp = to - (to @ d)[..., None] * d
+    dist = np.linalg.norm(perp, axis=-1)
+    w = np.where(dist < r, (1.0 - dist / r) ** 2 * k, 0.0)
+    edge = np.abs(dist - r) < 1e-4  # f32 may round either way at the rim
+    want = w[..., None] * vel
+    np.testing.assert_allclose(v[~edge], want[~edge], rtol=1e-5, atol=1e-6)
+    pushed = np.argwhere(np.abs(v).sum(-1) > 0)
+    assert len(pushed) > 100
+    lo = np.array([h.bbox_min[2], h.bbox_min[1], h.bbox_min[0]])
+    hi = np.array([h.bbox_max[2], h.bbox_max[1], h.bbox_max[0]])
+    assert (pushed >= lo).all() and (pushed <= hi).all()
+    # the line keeps z = 7.25: the box spans floor(7.25 - r) .. ceil(7.25 + r)
+    # in z, while the ray crosses the whole grid in x and y
+    assert (lo[0], hi[0]) == (4, 11)
+    assert (lo[1], hi[1], lo[2], hi[2]) == (0, g[1] - 1, 0, g[0] - 1)
