@@ -298,14 +298,19 @@ __device__ __forceinline__ bool brick_needed(const BrickWork &bw, uint32_t b) {
   if (b >= bw.nbricks) return false;
   int bx, by, bz;
   brick_coords(bw, b, bx, by, bz);
-  bool need = (bw.act[b] & 1) || bw.ofl[b];
-  for (int n = 0; n < 27 && !need; n++) {
-    int x = bx + n % 3 - 1, y = by + (n / 3) % 3 - 1, z = bz + n / 9 - 1;
-    if (x < 0 || y < 0 || z < 0 || x >= (int)bw.nbx || y >= (int)bw.nby || z >= (int)bw.nbz)
-      continue;
-    need = bw.occ[brick_index(bw, x, y, z)] != 0;
+  // All 27 neighbour flags load at once (indexes clamped into the grid, the
+  // outside ones masked), not one after another.
+  uint32_t need = (uint32_t)(bw.act[b] & 1) | (uint32_t)bw.ofl[b];
+#pragma unroll
+  for (int n = 0; n < 27; n++) {
+    const int x = bx + n % 3 - 1, y = by + (n / 3) % 3 - 1, z = bz + n / 9 - 1;
+    const bool in = x >= 0 && y >= 0 && z >= 0 && x < (int)bw.nbx && y < (int)bw.nby &&
+                    z < (int)bw.nbz;
+    const int cx = min(max(x, 0), (int)bw.nbx - 1), cy = min(max(y, 0), (int)bw.nby - 1),
+              cz = min(max(z, 0), (int)bw.nbz - 1);
+    need |= in ? (uint32_t)bw.occ[brick_index(bw, cx, cy, cz)] : 0u;
   }
-  return need;
+  return need != 0;
 }
 
 __device__ __forceinline__ void brick_list_one(const BrickWork &bw, uint32_t b) {
