@@ -120,3 +120,50 @@ def test_abi_struct_sizes():
     assert C.sizeof(capi.ServerConfig) == 48
     assert C.sizeof(capi.ServerStats) == 4 * 8 + 2 * 4 + 4 * 8 + 2 * 8
     assert C.sizeof(capi.RequestInfo) == 4 * 4 + 8 + 3 * 8
+
+
+# ctypes structure -> C type of include/mlsmpm.h
+ABI_STRUCTS = {"Config": "mpm_config", "BatchTuning": "mpm_batch_tuning",
+               "ServerConfig": "mpm_server_config", "ServerStats": "mpm_server_stats",
+               "RequestInfo": "mpm_request_info", "Hand": "mpm_hand", "Export": "mpm_export",
+               "Frame": "mpm_frame"}
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every field offset and size of the ctypes structures equals what a C
+    compiler makes of include/mlsmpm.h."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc is missing")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mlsmpm.h"', 'int main(void) {']
+    want = []
+    for py, c in ABI_STRUCTS.items():
+        S = getattr(capi, py)
+        lines.append(f'  printf("%zu\\n", sizeof({c}));')
+        want.append(C.sizeof(S))
+        for f in S._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({c}, {f[0]}));')
+            want.append(getattr(S, f[0]).offset)
+    lines.append('  return 0;\n}')
+    src, exe = tmp_path / "abi.c", tmp_path / "abi"
+    src.write_text("\n".join(lines))
+    r = subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert got == want
+
+
+def test_interaction_and_handoff_reject_null_handles():
+    L = capi.load()
+    v3 = (C.c_float * 3)(0, 0, 1)
+    t = C.c_uint64()
+    for rc in (L.mpm_set_hand(None, None), L.mpm_delete_near(None, v3, v3, 1.0, None),
+               L.mpm_raycast(None, v3, v3, 1.0, None, None),
+               L.mpm_append_particles(None, 1, v3, None), L.mpm_export_device(None, None, 0),
+               L.mpm_frame_begin(None, capi.MPM_EXPORT_POS, C.byref(t)),
+               L.mpm_frame_poll(None, 1, 0, None, None)):
+        assert rc == capi.MPM_ERR_INVALID
+    assert b"NULL" in L.mpm_last_error()
