@@ -99,7 +99,8 @@ def cpu_baseline(grid, pos, vel, seconds: float) -> dict:
     o.step(1, threads=threads)
     t1 = time.perf_counter()
     per = t1 - t0
-    more = max(0, min(20, int(seconds / max(per, 1e-6)) - 1))
+    # about `seconds` of CPU work in all (bounded at 200 steps)
+    more = max(0, min(199, int(seconds / max(per, 1e-6)) - 1))
     if more:
         o.step(more, threads=threads)
     t2 = time.perf_counter()
