@@ -247,3 +247,52 @@ def test_interaction_errors_and_batches():
         o.step(20)
         check_equal(s, o, "batched after append")
         b.remove(s)
+
+
+@pytest.mark.parametrize("mode", ["direct", "sorted8", "sorted1"])
+def test_hand_with_walls_and_diffuse_pool(mode):
+    """The hand push runs after the wall-aware grid update and before G2P and
+    the diffuse step: with a wall and the pool on, every grid, particle and
+    pool value stays bit-exact."""
+    from test_gpu_parity import make_pair, wall_block
+    pos = block(22)
+    walls = wall_block(2)
+    s, _ = make_pair(GRID, pos, D=8192, **MODES[mode])
+    s.set_walls(walls)
+    o = oracle.OracleScene(grid=GRID, diffuse_max=8192, seed=1, walls=walls)
+    o.reset(pos)
+    eye = (12.0, 30.0, -20.0)
+    for step in range(8):
+        both(s, o, "set_hand", (7.0 + 0.5 * step, 9.0, 8.0), (1.5, 2.0, 0.0), 3.0, 0.8, True,
+             eye)
+        s.step(1)
+        o.step(1)
+        check_equal(s, o, f"{mode} step {step}", diffuse=True)
+
+
+@pytest.mark.parametrize("mode", ["direct", "sorted1"])
+def test_delete_and_append_with_diffuse_pool(mode):
+    """The pool's spawners are found by reset-order id each step: after a
+    delete renumbers the survivors and an append adds ids, the pool, free
+    list and top stay bit-exact."""
+    pos = block(23)
+    cfg = mlsmpm.default_config(*GRID, len(pos) + 600, diffuse_capacity=8192, seed=1,
+                                **MODES[mode])
+    s = mlsmpm.Scene(cfg)
+    s.reset(pos)
+    o = oracle.OracleScene(grid=GRID, diffuse_max=8192, seed=1)
+    o.reset(pos)
+    s.step(4)
+    o.step(4)
+    check_equal(s, o, "before", diffuse=True)
+    both(s, o, "delete_near", (6.0, 0.0, 6.0), (0.0, 1.0, 0.0), 2.5)
+    for step in range(3):
+        s.step(1)
+        o.step(1)
+        check_equal(s, o, f"{mode} step {step} after delete", diffuse=True)
+    add = np.random.default_rng(24).uniform((3, 12, 3), (12, 18, 12), (600, 3)).astype(np.float32)
+    both(s, o, "append", add, np.tile(np.float32([0.0, -2.0, 0.0]), (600, 1)))
+    for step in range(4):
+        s.step(1)
+        o.step(1)
+        check_equal(s, o, f"{mode} step {step} after append", diffuse=True)
