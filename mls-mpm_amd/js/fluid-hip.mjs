@@ -78,6 +78,7 @@ export class FluidHIP {
         this._hand = null;          // last updateHand arguments, re-sent on a rebuild
         this.lastRayHit = null;
         this._deletePending = false;
+        this._frameTicket = null;
         this.device = null;
         this.onerror = null;
         this.lastError = null;
@@ -241,6 +242,7 @@ export class FluidHIP {
         this._scene = addon.create(cfg);
         this._sent = cfg;
         this._wallSent = null;
+        this._frameTicket = null;   // frames belonged to the previous scene
         const n = this.active_particle_num;
         const hand = this._hand;
         if (state) {
