@@ -93,3 +93,49 @@ def test_gpu_reproduces_dam16_with_diffuse(form):
         np.testing.assert_array_equal(fl, d["free_list"])
         for f in ("pos", "vel", "ptype", "alive", "lifetime"):
             np.testing.assert_array_equal(rec[f], d[f"pool_{f}"])
+
+
+def _interact16():
+    import sys
+    sys.path.insert(0, HERE)
+    import make_golden
+    return make_golden
+
+
+def _check_interact16(d, scene, mg):
+    def on_step(k):
+        if k in mg.KEPT_STEPS:
+            if isinstance(scene, oracle.OracleScene):
+                p4, v4 = scene.pos4, scene.vel4
+            else:
+                p = scene.read_particles(C9=False, density=False)
+                p4, v4 = p["pos4"], p["vel4"]
+            np.testing.assert_array_equal(p4, d[f"pos4_{k}"], err_msg=f"pos step {k}")
+            np.testing.assert_array_equal(v4, d[f"vel4_{k}"], err_msg=f"vel step {k}")
+
+    hit, t = mg.run_interact16(scene, d["pos3"], d["add3"], d["addv3"], on_step)
+    assert hit == bool(d["ray"][0]) and np.float32(t) == d["ray"][1]
+
+
+def test_oracle_reproduces_interact16():
+    d, mg = fixture("interact16"), _interact16()
+    o = oracle.OracleScene(grid=tuple(d["grid"]))
+    _check_interact16(d, o, mg)
+    np.testing.assert_array_equal(o.cell_mass, d["cell_mass"])
+    np.testing.assert_array_equal(o.cell_v, d["cell_v"])
+    np.testing.assert_array_equal(o.affine, d["C9"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", list(FORMS))
+def test_gpu_reproduces_interact16(form):
+    d, mg = fixture("interact16"), _interact16()
+    cap = len(d["pos3"]) + len(d["add3"])
+    with mlsmpm.Scene(mlsmpm.default_config(*d["grid"], cap, **FORMS[form])) as s:
+        _check_interact16(d, s, mg)
+        g, p = s.read_grid(), s.read_particles()
+        np.testing.assert_array_equal(g["cell_mass"], d["cell_mass"])
+        np.testing.assert_array_equal(g["cell_mv"], d["cell_mv"])
+        np.testing.assert_array_equal(g["cell_v"], d["cell_v"])
+        np.testing.assert_array_equal(p["C9"], d["C9"])
+        np.testing.assert_array_equal(p["density"], d["density"])
