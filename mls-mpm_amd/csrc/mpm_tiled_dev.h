@@ -92,15 +92,29 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
                                               const BrickWork &bw, const KeyOut &ko, uint32_t blk) {
   constexpr int E = 10 + 2 * M, V = E * E * E;
   __shared__ int32_t lds[V];
-  __shared__ uint32_t lcnt[27], lbase[27];
+  // the 27 neighbour bricks: particle counts, reserved bases, Morton ranks
+  __shared__ uint32_t lcnt[27], lbase[27], ldest[27];
+  // An item of at most kStageRanks particles keeps its (neighbour, rank)
+  // tags here until the bases are known; a larger one in ko.rank.
+  constexpr uint32_t kStageRanks = 8u * kTB;
+  __shared__ uint32_t lrk[kStageRanks];
   if (blk >= *bw.nitems) return;
   const uint4 it = bw.items[blk];
   int ibx, iby, ibz;
   brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
   const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
   const int t = threadIdx.x, nt = blockDim.x;
+  const bool stage = it.z <= kStageRanks;
   for (int c = t; c < V; c += nt) lds[c] = 0;
-  if (t < 27) lcnt[t] = 0;
+  if (t < 27) {
+    lcnt[t] = 0;
+    if (ko.counts) {
+      const int bx = ibx + t % 3 - 1, by = iby + (t / 3) % 3 - 1, bz = ibz + t / 9 - 1;
+      const bool in = bx >= 0 && by >= 0 && bz >= 0 && bx < (int)bw.nbx && by < (int)bw.nby &&
+                      bz < (int)bw.nbz;
+      ldest[t] = in ? bw.brick_rank[brick_index(bw, bx, by, bz)] : 0u;
+    }
+  }
   __syncthreads();
   for (uint32_t idx = t; idx < it.z; idx += nt) {
     uint32_t s = it.y + idx;
@@ -111,10 +125,12 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
       int bx, by, bz;
       cell_brick(P, px, py, pz, bx, by, bz);
       int dx = bx - ibx + 1, dy = by - iby + 1, dz = bz - ibz + 1;
-      uint32_t dest = bw.brick_rank[brick_index(bw, bx, by, bz)];
-      ko.keys[s] = dest;
       const bool near = dx >= 0 && dx < 3 && dy >= 0 && dy < 3 && dz >= 0 && dz < 3;
       const uint32_t nb = near ? (uint32_t)((dz * 3 + dy) * 3 + dx) : kFar;
+      // a near particle's brick is one of the 27 (inside the grid: cell_brick
+      // clamps), whose ranks are in LDS
+      const uint32_t dest = near ? ldest[nb] : bw.brick_rank[brick_index(bw, bx, by, bz)];
+      ko.keys[s] = dest;
       // Most particles stay in their own brick (nb 13): those of a
       // wavefront take consecutive ranks with one LDS atomic.
       const uint64_t own = __ballot(nb == 13u);
@@ -131,7 +147,8 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
       } else {
         r = atomicAdd(&ko.counts[dest], 1u);
       }
-      ko.rank[s] = (nb << 26) | r;
+      if (stage) lrk[idx] = (nb << 26) | r;
+      else ko.rank[s] = (nb << 26) | r;
     }
     float pm = UNIT ? 1.0f : S.at(PM, s);
     WinPos wp = win_pos<M>(pw, ox, oy, oz);
@@ -165,16 +182,13 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
   if (ko.counts) {
     if (t < 27) {
       uint32_t c = lcnt[t], base = 0;
-      if (c) {
-        int bx = ibx + t % 3 - 1, by = iby + (t / 3) % 3 - 1, bz = ibz + t / 9 - 1;
-        base = atomicAdd(&ko.counts[bw.brick_rank[brick_index(bw, bx, by, bz)]], c);
-      }
+      if (c) base = atomicAdd(&ko.counts[ldest[t]], c);
       lbase[t] = base;
     }
     __syncthreads();
     for (uint32_t idx = t; idx < it.z; idx += nt) {
       uint32_t s = it.y + idx;
-      uint32_t r = ko.rank[s], nb = r >> 26;
+      uint32_t r = stage ? lrk[idx] : ko.rank[s], nb = r >> 26;
       if (nb != kFar) ko.rank[s] = lbase[nb] + (r & 0x3ffffffu);
       else ko.rank[s] = r & 0x3ffffffu;
     }
