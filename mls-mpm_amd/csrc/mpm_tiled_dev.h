@@ -558,12 +558,31 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
   brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
   const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
   const int t = threadIdx.x, nt = blockDim.x;
-  for (int c = t; c < V; c += nt) {
-    int gi = window_cell<E>(P, ox, oy, oz, c);
-    lv[c] = gi >= 0 ? g.v[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  // Four loads in flight per thread, then their stores.
+  for (int c0 = t; c0 < V; c0 += 4 * nt) {
+    float4 v4[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int c = c0 + u * nt;
+      const int gi = c < V ? window_cell<E>(P, ox, oy, oz, c) : -1;
+      v4[u] = gi >= 0 ? g.v[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (c0 + u * nt < V) lv[c0 + u * nt] = v4[u];
   }
   if (EXACT)
-    for (uint32_t i = t; i < it.z; i += nt) lsrc[i] = src_of[it.y + i];
+    for (uint32_t i0 = t; i0 < it.z; i0 += 4u * nt) {
+      uint32_t s4[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t i = i0 + u * nt;
+        s4[u] = i < it.z ? src_of[it.y + i] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (i0 + u * nt < it.z) lsrc[i0 + u * nt] = s4[u];
+    }
   __syncthreads();
   uint32_t idx = t;
   if (idx >= it.z) return;
