@@ -101,7 +101,7 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
   if (blk >= *bw.nitems) return;
   const uint4 it = bw.items[blk];
   int ibx, iby, ibz;
-  brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
+  brick_coords(bw, it.w, ibx, iby, ibz);
   const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
   const int t = threadIdx.x, nt = blockDim.x;
   const bool stage = it.z <= kStageRanks;
@@ -225,7 +225,7 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
   if (blk >= *bw.nitems) return;
   const uint4 it = bw.items[blk];
   int ibx, iby, ibz;
-  brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
+  brick_coords(bw, it.w, ibx, iby, ibz);
   const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
   const int t = threadIdx.x, nt = blockDim.x;
   for (int c = t; c < V; c += nt) {
@@ -555,7 +555,7 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
   if (blk >= *(EXACT ? bw.nitems_w : bw.nitems)) return;
   const uint4 it = (EXACT ? bw.items_w : bw.items)[blk];
   int ibx, iby, ibz;
-  brick_coords(bw, bw.brick_of_rank[it.x], ibx, iby, ibz);
+  brick_coords(bw, it.w, ibx, iby, ibz);
   const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
   const int t = threadIdx.x, nt = blockDim.x;
   // Four loads in flight per thread, then their stores.
@@ -665,7 +665,7 @@ __device__ __forceinline__ void write_items(const BrickWork &bw, uint32_t r, uin
   bw.occ_w[b] = cnt ? 1 : 0;
   for (uint32_t p = 0; p < pieces; p++) {
     const uint32_t c = cnt - p * ch;
-    bw.items_w[item0 + p] = make_uint4(r, start + p * ch, c < ch ? c : ch, 0u);
+    bw.items_w[item0 + p] = make_uint4(r, start + p * ch, c < ch ? c : ch, b);
   }
 }
 
