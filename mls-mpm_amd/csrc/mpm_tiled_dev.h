@@ -14,6 +14,33 @@ constexpr int kTB = 256;    // row width of an item (slots per row)
 constexpr int kMaxPpt = 16; // largest particles-per-thread of a work item
 constexpr int kCore = 10;   // brick + 1-cell halo per axis
 constexpr int kCoreV = kCore * kCore * kCore;
+
+// Ghost core layout, kCoreV cells per item: the cells x = 1..8 of every
+// (z, y) row first, 8 contiguous per row, rows z-major (800 cells); then the
+// x = 0 plane and the x = 9 plane, (z, y)-major (100 cells each).  Every
+// region a neighbouring brick's resolve reads is then made of runs of at
+// least 8 contiguous cells; in x-fastest order its x faces would be one cell
+// every 10.
+constexpr int kGhostRows = kCore * kCore * (kCore - 2);
+__device__ __forceinline__ int ghost_off(int cx, int cy, int cz) {
+  const int yz = cz * kCore + cy;
+  return cx == 0 ? kGhostRows + yz
+                 : (cx == kCore - 1 ? kGhostRows + kCore * kCore + yz : yz * (kCore - 2) + cx - 1);
+}
+// The inverse: ghost index d to its core cell.
+__device__ __forceinline__ void ghost_cell(int d, int &cx, int &cy, int &cz) {
+  int yz;
+  if (d < kGhostRows) {
+    yz = d >> 3;
+    cx = (d & 7) + 1;
+  } else {
+    const int e = d - kGhostRows;
+    cx = e < kCore * kCore ? 0 : kCore - 1;
+    yz = e < kCore * kCore ? e : e - kCore * kCore;
+  }
+  cz = yz / kCore;
+  cy = yz - cz * kCore;
+}
 // Resolve blocks: cells per thread and threads per block (one 8^3 brick).
 constexpr int kResolveCells = 2;
 constexpr int kResolveThreads = 512 / kResolveCells;
@@ -196,7 +223,8 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
   // core to the ghost buffer, non-zero margin cells to the overflow grid
   int32_t *gh = bw.gm + (size_t)blk * kCoreV;
   for (int c = t; c < kCoreV; c += nt) {
-    int cz = c / 100, cy = (c / 10) % 10, cx = c % 10;
+    int cx, cy, cz;
+    ghost_cell(c, cx, cy, cz);
     gh[c] = lds[((cz + M) * E + (cy + M)) * E + (cx + M)];
   }
   if (M > 0) {
@@ -280,7 +308,8 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
   __syncthreads();
   const size_t gb = (size_t)blk * kCoreV;
   for (int c = t; c < kCoreV; c += nt) {
-    int cz = c / 100, cy = (c / 10) % 10, cx = c % 10;
+    int cx, cy, cz;
+    ghost_cell(c, cx, cy, cz);
     int w = ((cz + M) * E + (cy + M)) * E + (cx + M);
     bw.gx[gb + c] = lv[3 * w];
     bw.gy[gb + c] = lv[3 * w + 1];
@@ -434,7 +463,7 @@ __device__ __forceinline__ void gather_cells(const NbrItems &nb, const GhostSrc<
         }
       }
       tgt[u] = (l[2] * 8 + l[1]) * 8 + l[0];
-      src[u] = (size_t)(nb.first[n] + k) * kCoreV + (size_t)((cc[2] * kCore + cc[1]) * kCore + cc[0]);
+      src[u] = (size_t)(nb.first[n] + k) * kCoreV + (size_t)ghost_off(cc[0], cc[1], cc[2]);
     }
     int32_t v[4][R];
 #pragma unroll
