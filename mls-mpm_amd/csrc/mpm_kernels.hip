@@ -392,7 +392,7 @@ __global__ void k_kill_finish(int32_t *topp, const uint32_t *total) {
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of u32 counts (or of the popcounts of bitmask words):
-// tile reduce -> scan of tile sums -> tile scan with carry-in.
+// tile reduce, then the tile scan with the carry-in of the tiles before it.
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
@@ -440,39 +440,25 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *in
   if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-// One block scans all tile sums in place (exclusive) and stores the total.
-__global__ __launch_bounds__(kScanThreads) void k_scan_partials(uint32_t *partials, uint32_t nb,
-                                                               uint32_t *total) {
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < nb; base += kScanTile) {
-    uint32_t idx0 = base + threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
-#pragma unroll
-    for (int i = 0; i < kScanItems; i++) {
-      v[i] = (idx0 + i < nb) ? partials[idx0 + i] : 0u;
-      s += v[i];
-    }
-    uint32_t excl;
-    uint32_t tot = block_exclusive_scan(s, excl);
-    uint32_t run = carry + excl;
-#pragma unroll
-    for (int i = 0; i < kScanItems; i++) {
-      if (idx0 + i < nb) partials[idx0 + i] = run;
-      run += v[i];
-    }
-    carry += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && total) *total = carry;
-}
-
 // `clear` (the input array itself, or null) is zeroed behind the scan, which
-// leaves a histogram ready for its next use without a separate fill.
+// leaves a histogram ready for its next use without a separate fill.  Each
+// block adds up the tile sums before its own (one per thread for up to 256
+// tiles), and block 0 stores the total: no pass over the tile sums alone.
 template <bool POPC>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in, uint32_t n,
-                                                           const uint32_t *partials,
-                                                           uint32_t *out, uint32_t *clear) {
+                                                           const uint32_t *partials, uint32_t nb,
+                                                           uint32_t *out, uint32_t *clear,
+                                                           uint32_t *total) {
+  uint32_t before = 0, all = 0;
+  for (uint32_t i = threadIdx.x; i < nb; i += kScanThreads) {
+    const uint32_t p = partials[i];
+    all += p;
+    if (i < blockIdx.x) before += p;
+  }
+  uint32_t unused;
+  before = block_exclusive_scan(before, unused);
+  all = block_exclusive_scan(all, unused);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && total) *total = all;
   uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
@@ -483,7 +469,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in, 
   }
   uint32_t excl;
   block_exclusive_scan(s, excl);
-  uint32_t run = partials[blockIdx.x] + excl;
+  uint32_t run = before + excl;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
     if (base + i < n) {
@@ -593,9 +579,8 @@ void launch_scan(const uint32_t *in, uint32_t n, bool popc, uint32_t *out, uint3
   if (nb == 0) nb = 1;
   if (popc) k_scan_reduce<true><<<nb, kScanThreads, 0, st>>>(in, n, scratch);
   else k_scan_reduce<false><<<nb, kScanThreads, 0, st>>>(in, n, scratch);
-  k_scan_partials<<<1, kScanThreads, 0, st>>>(scratch, nb, total);
-  if (popc) k_scan_down<true><<<nb, kScanThreads, 0, st>>>(in, n, scratch, out, clear);
-  else k_scan_down<false><<<nb, kScanThreads, 0, st>>>(in, n, scratch, out, clear);
+  if (popc) k_scan_down<true><<<nb, kScanThreads, 0, st>>>(in, n, scratch, nb, out, clear, total);
+  else k_scan_down<false><<<nb, kScanThreads, 0, st>>>(in, n, scratch, nb, out, clear, total);
 }
 
 void launch_alive_bits(const DiffuseBufs &D, uint32_t dmax, hipStream_t st) {
