@@ -16,7 +16,7 @@ struct DiffuseBufs {
   uint32_t *alivebits; // alive flag of every slot (a copy of the records'), ceil(dmax/32) words
   uint32_t *prefix;    // scan output, max(ceil(cap/32), ceil(dmax/32)) words
   uint32_t *scratch;   // scan tile sums
-  uint32_t *total;     // scan total
+  uint32_t *total;     // [0] scan total, [1] stack top after the spawns
 };
 
 // RayParams (shaders.js:78): origin, cylinder radius, direction.

@@ -255,11 +255,18 @@ __global__ __launch_bounds__(256) void k_gen_apply(Params P, PState S, GState g,
 __global__ void k_zero_total(uint32_t *p) { *p = 0u; }
 
 // top -= min(spawners, top); the step's seed becomes the last-used seed.
-__global__ void k_gen_finish(int32_t *topp, const uint32_t *total, uint32_t *seedp) {
+// The stack after the spawns (the spawned count capped by the free slots)
+// and the next seed.  Run by one thread of the advect kernel that follows,
+// which reads neither; the new top is also kept in topsnap, the base the
+// kill push indexes from while it moves top on.
+__device__ __forceinline__ void gen_finish(int32_t *topp, const uint32_t *total, uint32_t *seedp,
+                                           int32_t *topsnap) {
   int32_t top = *topp;
   int32_t t = (int32_t)*total;
   int32_t avail = top > 0 ? top : 0;
-  *topp = top - (t < avail ? t : avail);
+  top -= t < avail ? t : avail;
+  *topp = top;
+  *topsnap = top;
   *seedp = *seedp + 1u;
 }
 
@@ -300,8 +307,11 @@ __device__ __forceinline__ void reflect_axis(float &p, float &v, float lo, float
 // The alive bitmask mirrors the records' alive flags, so dead slots are
 // skipped without touching the pool (32 slots per word read).
 __global__ __launch_bounds__(256) void k_advect(Params P, GState g, DRec *pool, uint32_t *killbits,
-                                                uint32_t *alivebits, Walls W) {
+                                                uint32_t *alivebits, Walls W, int32_t *topp,
+                                                const uint32_t *total, uint32_t *seedp,
+                                                int32_t *topsnap) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) gen_finish(topp, total, seedp, topsnap);
   if (i >= P.dmax) return;
   if (!((alivebits[i >> 5] >> (i & 31)) & 1u)) return;
   DRec p = pool[i];
@@ -369,15 +379,19 @@ __global__ __launch_bounds__(256) void k_alive_bits(const DRec *pool, uint32_t *
   alivebits[w] = bits;
 }
 
+// Pushes the killed slots in slot order above topsnap (the stack top after
+// the spawns); one thread moves top on by the number killed.
 __global__ __launch_bounds__(256) void k_kill_push(uint32_t *killbits, const uint32_t *prefix,
-                                                   uint32_t *free_list, const int32_t *topp,
+                                                   uint32_t *free_list, const int32_t *topsnap,
+                                                   int32_t *topp, const uint32_t *total,
                                                    uint32_t nwords, uint32_t dmax) {
   uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w == 0) *topp = *topsnap + (int32_t)*total;
   if (w >= nwords) return;
   uint32_t word = killbits[w];
   if (!word) return;
   killbits[w] = 0;
-  int32_t pos = *topp + (int32_t)prefix[w];
+  int32_t pos = *topsnap + (int32_t)prefix[w];
   while (word) {
     int b = __ffs(word) - 1;
     word &= word - 1;
@@ -386,9 +400,6 @@ __global__ __launch_bounds__(256) void k_kill_push(uint32_t *killbits, const uin
   }
 }
 
-__global__ void k_kill_finish(int32_t *topp, const uint32_t *total) {
-  *topp = *topp + (int32_t)*total;
-}
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of u32 counts (or of the popcounts of bitmask words):
@@ -599,13 +610,13 @@ void launch_diffuse(const Params &P, const PState &S, const GState &g, const Wal
   } else {
     k_zero_total<<<1, 1, 0, st>>>(D.total);
   }
-  k_gen_finish<<<1, 1, 0, st>>>(D.top, D.total, D.seed);
-  k_advect<<<blocks(P.dmax, 256), 256, 0, st>>>(P, g, D.pool, D.killbits, D.alivebits, W);
+  int32_t *topsnap = (int32_t *)(D.total + 1);
+  k_advect<<<blocks(P.dmax, 256), 256, 0, st>>>(P, g, D.pool, D.killbits, D.alivebits, W, D.top,
+                                               D.total, D.seed, topsnap);
   uint32_t nwk = blocks(P.dmax, 32);
   launch_scan(D.killbits, nwk, true, D.prefix, D.scratch, D.total, st);
-  k_kill_push<<<blocks(nwk, 256), 256, 0, st>>>(D.killbits, D.prefix, D.free_list, D.top, nwk,
-                                               P.dmax);
-  k_kill_finish<<<1, 1, 0, st>>>(D.top, D.total);
+  k_kill_push<<<blocks(nwk, 256), 256, 0, st>>>(D.killbits, D.prefix, D.free_list, topsnap, D.top,
+                                               D.total, nwk, P.dmax);
 }
 
 void launch_export_particles(const PState &S, uint32_t n, float4 *pos4, float4 *vel4, float *C9,

@@ -345,7 +345,7 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     DALLOC(s->d.alivebits, (size_t)nwk * 4);
     DALLOC(s->d.prefix, (size_t)nw * 4);
     DALLOC(s->d.scratch, (size_t)scan_scratch_words(nw) * 4);
-    DALLOC(s->d.total, 4);
+    DALLOC(s->d.total, 8);  // scan total; the stack top after the spawns
     HIPCHK(hipMemsetAsync(s->d.genbits, 0, (size_t)nwg * 4, s->stream));
     HIPCHK(hipMemsetAsync(s->d.killbits, 0, (size_t)nwk * 4, s->stream));
   }
