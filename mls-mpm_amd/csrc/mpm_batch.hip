@@ -92,25 +92,31 @@ __global__ __launch_bounds__(256) void k_batch_brick_list(const SceneDev *__rest
   }
 }
 
-__global__ __launch_bounds__(kResolveThreads) void k_batch_resolve_mass(
+// The batched resolves walk tens of bricks per block, each a chain of
+// dependent loads: 128-thread blocks, four cells per thread, keep twice as
+// many chains in flight per CU as the single-scene shape.
+constexpr int kBatchResolveCells = 4;
+constexpr int kBatchResolveThreads = 512 / kBatchResolveCells;
+
+__global__ __launch_bounds__(kBatchResolveThreads) void k_batch_resolve_mass(
     const SceneDev *__restrict__ sd, const uint2 *__restrict__ list, const uint32_t *nlist) {
   const uint32_t nl = *nlist;
   for (uint32_t u = blockIdx.x; u < nl; u += gridDim.x) {
     const uint2 e = list[u];
     const uint32_t si = __builtin_amdgcn_readfirstlane(e.x), b = __builtin_amdgcn_readfirstlane(e.y);
     const SceneDev &d = sd[si];
-    resolve_mass_entry<kResolveCells>(d.P, d.g, d.bw, b);
+    resolve_mass_entry<kBatchResolveCells>(d.P, d.g, d.bw, b);
   }
 }
 
-__global__ __launch_bounds__(kResolveThreads) void k_batch_resolve_update(
+__global__ __launch_bounds__(kBatchResolveThreads) void k_batch_resolve_update(
     const SceneDev *__restrict__ sd, const uint2 *__restrict__ list, const uint32_t *nlist) {
   const uint32_t nl = *nlist;
   for (uint32_t u = blockIdx.x; u < nl; u += gridDim.x) {
     const uint2 e = list[u];
     const uint32_t si = __builtin_amdgcn_readfirstlane(e.x), b = __builtin_amdgcn_readfirstlane(e.y);
     const SceneDev &d = sd[si];
-    resolve_update_entry<kResolveCells>(d.P, d.g, d.bw, d.W, d.keep, b);
+    resolve_update_entry<kBatchResolveCells>(d.P, d.g, d.bw, d.W, d.keep, b);
   }
 }
 
@@ -166,10 +172,10 @@ int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &gr
   e = hipEventRecord(join, side);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step join");
   k_batch_brick_list<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks, t.list, t.nlist);
-  k_batch_resolve_mass<<<gr, kResolveThreads, 0, st>>>(t.sd, t.list, t.nlist);
+  k_batch_resolve_mass<<<gr, kBatchResolveThreads, 0, st>>>(t.sd, t.list, t.nlist);
   MPM_BATCH_ITEMS(k_batch_mom);
 #undef MPM_BATCH_ITEMS
-  k_batch_resolve_update<<<gr, kResolveThreads, 0, st>>>(t.sd, t.list, t.nlist);
+  k_batch_resolve_update<<<gr, kBatchResolveThreads, 0, st>>>(t.sd, t.list, t.nlist);
   e = hipStreamWaitEvent(st, join, 0);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step wait");
   k_batch_g2p<<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items, t.nlist);
@@ -416,7 +422,7 @@ int mpm_batch_create(int device, mpm_batch **out) {
   mpm_batch *b = new mpm_batch();
   b->device = device;
   b->tune.item_blocks = 0;
-  b->tune.resolve_blocks = 2048;
+  b->tune.resolve_blocks = 4096;  // 16 resident blocks of 128 threads per CU
   b->tune.aux_blocks = 2048;
   b->tune.graph_steps = 16;
   hipError_t ce = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
