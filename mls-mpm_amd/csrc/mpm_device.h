@@ -46,8 +46,17 @@ struct PState {
   uint32_t *id;      // reset-order index of the particle held in each slot
   float *density;    // particle_density, by slot
   uint32_t cap;
+  // A slot's byte offset fits in 32 bits (capacity <= 2^26): a uniform
+  // plane base plus a 32-bit lane offset, which the compiler can address as
+  // a scalar base and one vector offset instead of a 64-bit address per plane.
   __device__ __forceinline__ float &at(int k, uint32_t p) const {
-    return pl[(size_t)k * cap + p];
+    return *reinterpret_cast<float *>(reinterpret_cast<char *>(pl + (size_t)k * cap) + (p << 2));
+  }
+  __device__ __forceinline__ uint32_t &idr(uint32_t p) const {
+    return *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(id) + (p << 2));
+  }
+  __device__ __forceinline__ float &dens(uint32_t p) const {
+    return *reinterpret_cast<float *>(reinterpret_cast<char *>(density) + (p << 2));
   }
 };
 

@@ -117,7 +117,7 @@ __device__ __forceinline__ void p2g_mom_particle(const Params &P, const PState &
                                                  const PW &pw, MassAt mass_at, Emit emit) {
   MomIn in;
   load_mom_in<UNIT>(S, p, in);
-  S.density[p] = p2g_mom_body<UNIT>(P, in, pw, mass_at, emit);
+  S.dens(p) = p2g_mom_body<UNIT>(P, in, pw, mass_at, emit);
 }
 
 // Grid update of one cell (WGSL_UPDATE_GRID, shaders.js:443-512).  Returns

@@ -301,7 +301,7 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
             atomicAdd(&bw.oz[c], ez);
           });
     }
-    S.density[s] = density;
+    S.dens(s) = density;
     cur = nxt;
     idx = nidx;
   }
@@ -635,8 +635,8 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
     in.pz = S.at(PZ, in.s);
     if (EXACT) {
       in.m = P.unit_mass ? 1.0f : S.at(PM, in.s);  // unit mass: the plane is never read
-      in.id = S.id[in.s];
-      in.dens = S.density[in.s];
+      in.id = S.idr(in.s);
+      in.dens = S.dens(in.s);
     }
   };
   auto body = [&](uint32_t i, const In &cur) {
@@ -662,8 +662,8 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
     store_g2p(D, d, o);
     if (EXACT) {
       if (!P.unit_mass) D.at(PM, d) = cur.m;
-      D.id[d] = cur.id;
-      D.density[d] = cur.dens;
+      D.idr(d) = cur.id;
+      D.dens(d) = cur.dens;
     }
   };
   In cur, nxt;
