@@ -22,7 +22,10 @@ Prints one JSON line:
                    peak (active cells: the mean of ten samples over that run); `traffic` is
                    the HBM bytes per launch that rocprofv3 FETCH_SIZE +
                    WRITE_SIZE read for that kernel (profiles/traffic.json),
-                   scaled by the same counters' reading for a plain copy
+                   scaled by the same counters' reading for a plain copy;
+                   `valu` is that kernel's floor at the exact-FP32 VALU issue
+                   rate (SQ_INSTS_VALU, profiles/valu.json) over the same
+                   launch time: the bound that holds it, as HBM does not
   cpu_baseline     the CPU restatement of the reference step on this host's
                    cores, on a bounded sample of the same workload
 Extra keys: per-stage mean ms, whole-step roofline fraction, p50/p99 step
@@ -445,6 +448,16 @@ def main() -> int:
         ent = tr.get(args.workload, {}).get(dom)
         if ent:
             traffic = ent.get("bytes_per_launch")
+    # the same kernel's VALU floor (exact FP32 issue rate) from the committed
+    # counter summary: what bounds it, beside the HBM fraction
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "valu.json")
+    if os.path.exists(vpath):
+        with open(vpath) as f:
+            ve = json.load(f).get(args.workload, {}).get(dom)
+        if ve and ms[dom] > 0:
+            valu = {"floor_ms": ve["floor_ms"], "frac": ve["floor_ms"] / ms[dom],
+                    "insts_per_launch": ve["insts_valu"]}
     d_alive = s.diffuse_alive() if D else 0
     # diffuse term: density read per particle, alive flag per slot, 72 B per
     # alive record (spawn and kill traffic left out)
@@ -485,6 +498,7 @@ def main() -> int:
             "traffic": traffic,
             "alg_bytes_per_launch": alg,
             "launch_ms": ms[dom],
+            "valu": valu,
         },
         "step_roofline_frac": step_alg / (step_ms_mean * 1e-3) / HBM_PEAK,
         "stage_ms": ms,

@@ -106,18 +106,41 @@ def traffic(a) -> dict:
         "scale_write": sw,
         "bytes_per_launch": fetch * sf + write * sw,
     }
-    out = {}
     if a.out:
-        try:
-            with open(a.out) as f:
-                out = json.load(f)
-        except FileNotFoundError:
-            out = {}
-        out.setdefault(a.workload, {})[a.stage] = ent
-        with open(a.out, "w") as f:
-            json.dump(out, f, indent=1, sort_keys=True)
-            f.write("\n")
+        _merge(a.out, a.workload, a.stage, ent)
     return ent
+
+
+# Exact FP32 retires one wave64 VALU instruction per SIMD every 4 clocks:
+# the VALU floor of a launch is INSTS_VALU * 4 / (SIMDs * clock).
+MI355X_SIMDS = 1024
+MI355X_CLOCK_HZ = 2.4e9
+
+
+def valu(a) -> dict:
+    insts = _one(a.db, a.kernel, "SQ_INSTS_VALU")
+    ent = {
+        "kernel": a.kernel,
+        "insts_valu": insts,
+        "simds": MI355X_SIMDS,
+        "clock_hz": MI355X_CLOCK_HZ,
+        "floor_ms": insts * 4.0 / (MI355X_SIMDS * MI355X_CLOCK_HZ) * 1e3,
+    }
+    if a.out:
+        _merge(a.out, a.workload, a.stage, ent)
+    return ent
+
+
+def _merge(path: str, workload: str, stage: str, ent: dict) -> None:
+    try:
+        with open(path) as f:
+            out = json.load(f)
+    except FileNotFoundError:
+        out = {}
+    out.setdefault(workload, {})[stage] = ent
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+        f.write("\n")
 
 
 def main(argv=None) -> int:
@@ -138,6 +161,12 @@ def main(argv=None) -> int:
     t.add_argument("--stage", required=True)
     t.add_argument("--workload", required=True)
     t.add_argument("--out")
+    v = sub.add_parser("valu")
+    v.add_argument("--db", required=True)
+    v.add_argument("--kernel", required=True)
+    v.add_argument("--stage", required=True)
+    v.add_argument("--workload", required=True)
+    v.add_argument("--out")
     a = ap.parse_args(argv)
     if a.cmd == "kt":
         for db in a.db:
@@ -145,6 +174,8 @@ def main(argv=None) -> int:
             print_kernels(db, a.last)
     elif a.cmd == "pmc":
         print_counters(a.db, a.match or None)
+    elif a.cmd == "valu":
+        print(json.dumps(valu(a), indent=1))
     else:
         print(json.dumps(traffic(a), indent=1))
     return 0
