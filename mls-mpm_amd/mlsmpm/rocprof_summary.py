@@ -13,6 +13,10 @@
         HBM bytes per launch of one kernel from FETCH_SIZE and WRITE_SIZE
         (KiB per dispatch), each scaled by known/measured for a plain copy
         kernel of B bytes read and B bytes written.
+    python -m mlsmpm.rocprof_summary valu --db A.db --kernel K --stage S \
+        --workload NAME [--out profiles/valu.json]
+        the kernel's floor at the exact-FP32 VALU issue rate from
+        SQ_INSTS_VALU per dispatch.
 """
 from __future__ import annotations
 
