@@ -162,7 +162,8 @@ void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, co
 
 void launch_resolve_mass(const Params &P, const GState &g, const BrickWork &bw, hipStream_t st) {
   if (!P.n) return;
-  k_brick_list<<<nblk(bw.nbricks, 256), 256, 0, st>>>(bw);
+  // 64-thread blocks: one brick per thread spread over more CUs
+  k_brick_list<<<nblk(bw.nbricks, 64), 64, 0, st>>>(bw);
   k_resolve_mass<<<resolve_grid(bw), kResolveThreads, 0, st>>>(P, g, bw);
 }
 
