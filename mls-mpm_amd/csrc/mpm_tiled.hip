@@ -48,9 +48,19 @@ __global__ __launch_bounds__(256) void k_p2g_mom_brick(Params P, PState S, GStat
   p2g_mom_item<M, UNIT>(P, S, g, bw, blockIdx.x);
 }
 
-__global__ __launch_bounds__(256) void k_brick_list(BrickWork bw) {
-  brick_list_one(bw, blockIdx.x * blockDim.x + threadIdx.x);
+// One wavefront per 64 bricks: its needed bricks go to the list together,
+// in brick order, with one atomic (the list follows the grid).
+__global__ __launch_bounds__(64) void k_brick_list(BrickWork bw) {
+  const uint32_t lane = threadIdx.x, b = blockIdx.x * 64u + lane;
+  const bool need = brick_needed(bw, b);
+  const uint64_t m = __ballot(need);
+  if (!m) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(bw.nlist, (uint32_t)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (need) bw.list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = b;
 }
+
 
 template <int M, bool EXACT>
 __global__ __launch_bounds__(256) void k_g2p_brick(Params P, PState S, PState D, GState g,

@@ -356,9 +356,6 @@ __device__ __forceinline__ bool brick_needed(const BrickWork &bw, uint32_t b) {
   return need != 0;
 }
 
-__device__ __forceinline__ void brick_list_one(const BrickWork &bw, uint32_t b) {
-  if (brick_needed(bw, b)) bw.list[atomicAdd(bw.nlist, 1u)] = b;
-}
 
 struct NbrItems {
   uint32_t first[27], count[27];
