@@ -86,24 +86,26 @@ def workload(name: str, seed: int):
     return grid, pos, vel, desc
 
 
-def cpu_baseline(grid, pos, vel, seconds: float) -> dict:
+def cpu_baseline(grid, pos, vel, seconds: float, max_threads: int = 16,
+                 max_steps: int = 200) -> dict:
     """The oracle (C restatement of the reference kernels, OpenMP) on a
-    bounded number of steps of the same scene."""
+    bounded number of steps of the same scene, on up to `max_threads` host
+    threads."""
     from oracle import oracle
 
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    threads = max(1, min(16, cores))
+    threads = max(1, min(max_threads, cores))
     o = oracle.OracleScene(grid=tuple(grid))
     o.reset(pos, vel)
     t0 = time.perf_counter()
     o.step(1, threads=threads)
     t1 = time.perf_counter()
     per = t1 - t0
-    # about `seconds` of CPU work in all (bounded at 200 steps)
-    more = max(0, min(199, int(seconds / max(per, 1e-6)) - 1))
+    # about `seconds` of CPU work in all (bounded at max_steps steps)
+    more = max(0, min(max_steps - 1, int(seconds / max(per, 1e-6)) - 1))
     if more:
         o.step(more, threads=threads)
     t2 = time.perf_counter()
@@ -345,6 +347,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the CPU baseline (at most the cores available)")
+    ap.add_argument("--cpu-max-steps", type=int, default=200)
     ap.add_argument("--no-events", action="store_true",
                     help="skip the event-timed region (no stage times, no roofline)")
     ap.add_argument("--batch-scenes", type=int, default=256)
@@ -520,7 +525,8 @@ def main() -> int:
         line["served"] = serve_bench(args)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(grid, pos, vel, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(grid, pos, vel, args.cpu_seconds, args.cpu_threads,
+                                            args.cpu_max_steps)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
