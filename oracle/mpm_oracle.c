@@ -468,6 +468,31 @@ static int estimate_normal(const orc_params *P, const real *cmf, real px, real p
     return 1;
 }
 
+/* The spawn potential of one fluid particle (WGSL_DIFFUSE_GENERATE,
+ * shaders.js:803-826): kinetic energy, trapped air from the EOS pressure and
+ * the wave crest from the mass-gradient normal.  terms = (ke, trappedAir,
+ * waveCrest, speed). */
+static real spawn_score(const orc_params *P, const orc_gen *gen, const real *cmf, v3 pos, v3 vel,
+                        real mass, real density, real terms[4]) {
+    real speed2 = (vel.x * vel.x + vel.y * vel.y) + vel.z * vel.z;
+    real speed = SQRTR(speed2);
+    real ke = (K(0.5) * mass) * speed2;
+    real pressure = eos_pressure(P, density);
+    real trappedAir = MAXR(K(0.0), pressure - (real)gen->pressure_threshold);
+    v3 nrm;
+    int valid = estimate_normal(P, cmf, pos.x, pos.y, pos.z, &nrm);
+    real waveCrest = K(0.0);
+    if (speed > K(1e-4) && valid) {
+        real dotVal = ((vel.x / speed) * nrm.x + (vel.y / speed) * nrm.y) + (vel.z / speed) * nrm.z;
+        if (dotVal > (real)gen->crest_dot_threshold) waveCrest = dotVal * speed;
+    }
+    terms[0] = ke;
+    terms[1] = trappedAir;
+    terms[2] = waveCrest;
+    terms[3] = speed;
+    return (ke + trappedAir * K(2.0)) + waveCrest * K(1.5);
+}
+
 /* WGSL_DIFFUSE_GENERATE (shaders.js:721-856).  Particles are visited in index
  * order, so the free-list pops (try_spawn, shaders.js:791-798) happen in
  * particle-index order: the serialization the HIP library also uses. */
@@ -480,21 +505,11 @@ void orc_diffuse_generate(const orc_params *P, const orc_gen *gen, const real *p
         real density = density_in[p];
         if (density <= K(0.0) || density > (real)gen->density_bubble_min) continue;
         real vx = vel4[4 * p], vy = vel4[4 * p + 1], vz = vel4[4 * p + 2];
-        real speed2 = (vx * vx + vy * vy) + vz * vz;
-        real speed = SQRTR(speed2);
-        real ke = (K(0.5) * mass[p]) * speed2;
-        real pressure = eos_pressure(P, density);
-        real trappedAir = MAXR(K(0.0), pressure - (real)gen->pressure_threshold);
-
         real px = pos4[4 * p], py = pos4[4 * p + 1], pz = pos4[4 * p + 2];
-        v3 nrm;
-        int valid = estimate_normal(P, cmf, px, py, pz, &nrm);
-        real waveCrest = K(0.0);
-        if (speed > K(1e-4) && valid) {
-            real dotVal = ((vx / speed) * nrm.x + (vy / speed) * nrm.y) + (vz / speed) * nrm.z;
-            if (dotVal > (real)gen->crest_dot_threshold) waveCrest = dotVal * speed;
-        }
-        real score = (ke + trappedAir * K(2.0)) + waveCrest * K(1.5);
+        real terms[4];
+        real score = spawn_score(P, gen, cmf, mk3(px, py, pz), mk3(vx, vy, vz), mass[p], density,
+                                 terms);
+        real speed = terms[3];
         real keth = (real)gen->ke_threshold;
         if (score < keth) continue;
 
@@ -793,4 +808,38 @@ void orc_step(const orc_params *P, orc_state *S, uint32_t nsteps) {
             orc_diffuse_advect(P, &S->adv, S->pool, S->cell_v, S->free_list, S->top, S->wallp);
         }
     }
+}
+
+/* ---------------------------------------------------------------------------
+ * Single helpers exported for the closed-form pins of the tests: each wraps
+ * the function the kernels above call, unchanged. */
+void orc_wall_to_local(const real p[3], const float *wall, real out[3]) {
+    v3 r = wall_to_local(mk3(p[0], p[1], p[2]), wall);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void orc_wall_to_world_dir(const real v[3], const float *wall, real out[3]) {
+    v3 r = wall_to_world_dir(mk3(v[0], v[1], v[2]), wall);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void orc_wall_vel_at(const real p[3], const float *wall, real out[3]) {
+    v3 r = wall_vel_at(mk3(p[0], p[1], p[2]), wall);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+real orc_wall_sdf_normal(const real lp[3], const real mn[3], const real mx[3], real n[3]) {
+    v3 nn;
+    real d = wall_sdf_normal(mk3(lp[0], lp[1], lp[2]), mk3(mn[0], mn[1], mn[2]),
+                             mk3(mx[0], mx[1], mx[2]), &nn);
+    n[0] = nn.x; n[1] = nn.y; n[2] = nn.z;
+    return d;
+}
+int orc_estimate_normal(const orc_params *P, const real *cell_mass_f, const real p[3], real n[3]) {
+    v3 nn = mk3(K(0.0), K(0.0), K(0.0));
+    int ok = estimate_normal(P, cell_mass_f, p[0], p[1], p[2], &nn);
+    n[0] = nn.x; n[1] = nn.y; n[2] = nn.z;
+    return ok;
+}
+real orc_spawn_score(const orc_params *P, const orc_gen *gen, const real *cell_mass_f,
+                     const real pos[3], const real vel[3], real mass, real density, real terms[4]) {
+    return spawn_score(P, gen, cell_mass_f, mk3(pos[0], pos[1], pos[2]),
+                       mk3(vel[0], vel[1], vel[2]), mass, density, terms);
 }

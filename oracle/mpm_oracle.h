@@ -138,6 +138,19 @@ orc_real orc_ray_perp_dist(const orc_real *pos4, const orc_ray *r);
 int orc_raycast(const orc_real *pos4, uint32_t n, const orc_ray *r, orc_real *t);
 uint32_t orc_delete_mask(const orc_real *pos4, uint32_t n, const orc_ray *r, uint8_t *keep);
 void orc_step(const orc_params *P, orc_state *S, uint32_t nsteps);
+/* Helpers of the kernels above, exported for the tests' closed-form pins
+ * (wall = 20 floats of one wall, shaders.js:123-129). */
+void orc_wall_to_local(const orc_real p[3], const float *wall, orc_real out[3]);
+void orc_wall_to_world_dir(const orc_real v[3], const float *wall, orc_real out[3]);
+void orc_wall_vel_at(const orc_real p[3], const float *wall, orc_real out[3]);
+orc_real orc_wall_sdf_normal(const orc_real lp[3], const orc_real mn[3], const orc_real mx[3],
+                             orc_real n[3]);
+int orc_estimate_normal(const orc_params *P, const orc_real *cell_mass_f, const orc_real p[3],
+                        orc_real n[3]);
+/* terms = (ke, trappedAir, waveCrest, speed); returns the spawn score. */
+orc_real orc_spawn_score(const orc_params *P, const orc_gen *gen, const orc_real *cell_mass_f,
+                         const orc_real pos[3], const orc_real vel[3], orc_real mass,
+                         orc_real density, orc_real terms[4]);
 #ifdef __cplusplus
 }
 #endif

@@ -116,6 +116,15 @@ class _Lib:
         L.orc_raycast.restype = C.c_int
         L.orc_delete_mask.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
         L.orc_delete_mask.restype = C.c_uint32
+        for name in ("orc_wall_to_local", "orc_wall_to_world_dir", "orc_wall_vel_at"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_wall_sdf_normal.argtypes = [C.c_void_p] * 4
+        L.orc_wall_sdf_normal.restype = ct
+        L.orc_estimate_normal.argtypes = [C.c_void_p] * 4
+        L.orc_estimate_normal.restype = C.c_int
+        L.orc_spawn_score.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      ct, ct, C.c_void_p]
+        L.orc_spawn_score.restype = ct
         self.bw_dtype = np.dtype([("b", np.int32, 3), ("f", self.real, 3), ("w", self.real, (3, 3))],
                                  align=True)
         self.drec_dtype = np.dtype([("pos", self.real, 4), ("vel", self.real, 4), ("ptype", np.uint32),
@@ -360,3 +369,66 @@ def decode(i: int, real: str = "f32") -> float:
 
 def diffuse_rand(seed: int) -> float:
     return lib("f32").lib.orc_diffuse_rand(seed)
+
+
+# --- single helpers of the kernels, for the closed-form pins ---------------
+def make_wall(center, lmin, lmax, yaw=0.0, pitch=0.0, vel=(0, 0, 0), omega=(0, 0, 0)) -> np.ndarray:
+    """One wall's 20 floats (shaders.js:123-129): center + cos(yaw), local min
+    + sin(yaw), local max + cos(pitch), velocity + sin(pitch), omega."""
+    w = np.zeros(20, np.float32)
+    w[0:3], w[3] = center, np.cos(yaw)
+    w[4:7], w[7] = lmin, np.sin(yaw)
+    w[8:11], w[11] = lmax, np.cos(pitch)
+    w[12:15], w[15] = vel, np.sin(pitch)
+    w[16:19] = omega
+    return w
+
+
+def wall_params(*walls) -> np.ndarray:
+    """The WallParams uniform: 4 header floats (count in [0]) + 4 walls."""
+    wp = np.zeros(4 + 4 * 20, np.float32)
+    wp[0] = len(walls)
+    for i, w in enumerate(walls):
+        wp[4 + 20 * i:24 + 20 * i] = w
+    return wp
+
+
+def _vec(v, R):
+    return np.ascontiguousarray(np.asarray(v, dtype=R).reshape(3))
+
+
+def wall_helper(name: str, v, wall, real: str = "f32") -> np.ndarray:
+    """wall_to_local / wall_to_world_dir / wall_vel_at of one wall."""
+    L = lib(real)
+    out = np.zeros(3, L.real)
+    w = np.ascontiguousarray(wall, dtype=np.float32)
+    getattr(L.lib, "orc_" + name)(_ptr(_vec(v, L.real)), _ptr(w), _ptr(out))
+    return out
+
+
+def wall_sdf_normal(lp, mn, mx, real: str = "f32"):
+    L = lib(real)
+    n = np.zeros(3, L.real)
+    d = L.lib.orc_wall_sdf_normal(_ptr(_vec(lp, L.real)), _ptr(_vec(mn, L.real)),
+                                  _ptr(_vec(mx, L.real)), _ptr(n))
+    return float(d), n
+
+
+def estimate_normal(params: Params, cell_mass_f: np.ndarray, p, real: str = "f32"):
+    L = lib(real)
+    n = np.zeros(3, L.real)
+    cm = np.ascontiguousarray(cell_mass_f, dtype=L.real)
+    ok = L.lib.orc_estimate_normal(C.byref(params), _ptr(cm), _ptr(_vec(p, L.real)), _ptr(n))
+    return bool(ok), n
+
+
+def spawn_score(params: Params, gen: Gen, cell_mass_f: np.ndarray, pos, vel, mass: float,
+                density: float, real: str = "f32"):
+    """(score, {ke, trapped, crest, speed}) of one fluid particle."""
+    L = lib(real)
+    t = np.zeros(4, L.real)
+    cm = np.ascontiguousarray(cell_mass_f, dtype=L.real)
+    sc = L.lib.orc_spawn_score(C.byref(params), C.byref(gen), _ptr(cm), _ptr(_vec(pos, L.real)),
+                               _ptr(_vec(vel, L.real)), mass, density, _ptr(t))
+    return float(sc), {"ke": float(t[0]), "trapped": float(t[1]), "crest": float(t[2]),
+                       "speed": float(t[3])}
