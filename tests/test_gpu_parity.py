@@ -156,6 +156,25 @@ def test_diffuse_pool_saturation(mode):
         check_equal(s, o, f"step {step}", diffuse=True)
 
 
+@pytest.mark.parametrize("mode", ["direct", "sorted1"])
+def test_diffuse_trapped_air_spawns_match_oracle(mode):
+    """density_bubble_min raised above the rest density: particles of the
+    compressed dam-break core pass the density window, so the trapped-air
+    term of the spawn score (shaders.js:813-816) decides spawns too."""
+    grid = (32, 32, 32)
+    pos = scenes.lattice_block((2, 2, 2), (14, 22, 14), 0.1, np.random.default_rng(6))
+    s, o = make_pair(grid, pos, D=40000, density_bubble_min=12.0, **MODES[mode])
+    o.gen.density_bubble_min = 12.0
+    dens_hi = 0
+    for step in range(12):
+        s.step(1)
+        o.step(1)
+        check_equal(s, o, f"step {step}", diffuse=True)
+        dens_hi = max(dens_hi, int(np.count_nonzero(o.density > 9.0)))
+    assert dens_hi > 0          # the window admitted particles whose pressure exceeds 1
+    s.close()
+
+
 @pytest.mark.parametrize("mode", ["direct", "sorted8"])
 def test_non_unit_mass(mode):
     grid = (28, 28, 28)
@@ -334,6 +353,36 @@ def test_reset_keeps_graphs_and_bits(period):
     g.close()
     f.close()
     f2.close()
+
+
+@pytest.mark.parametrize("period,timed", [(1, False), (8, False), (1, True)])
+def test_prepare_captures_graphs_the_run_replays(period, timed):
+    """mpm_prepare right after a reset (and again mid-run) captures without
+    stepping: the steps that follow equal eager steps and the oracle."""
+    grid = (40, 32, 36)
+    pos = scenes.lattice_block((2, 2, 2), (22, 22, 20), 0.1, np.random.default_rng(13))
+    kw = dict(flags=KEEP, sort_period=period, tile_ppt=4)
+    g, o = make_pair(grid, pos, D=20000, **kw)
+    e, _ = make_pair(grid, pos, D=20000, flags=KEEP | mlsmpm.MPM_FLAG_NO_GRAPH,
+                     sort_period=period, tile_ppt=4)
+    if timed:
+        g.enable_timing(True)
+    g.prepare()
+    assert g.counts()[1] == 0          # nothing ran
+    check_equal(g, o, "after prepare", diffuse=True)
+    g.step(20)
+    g.prepare()
+    g.step(19)
+    e.step(39)
+    o.step(39)
+    check_equal(g, o, "prepared run vs oracle after 39 steps", diffuse=True)
+    pg, pe = g.read_particles(), e.read_particles()
+    for k in pg:
+        np.testing.assert_array_equal(pg[k], pe[k], err_msg=f"prepared vs eager {k}")
+    if timed:
+        assert len(g.step_times()) == 39
+    g.close()
+    e.close()
 
 
 def test_state_round_trip_with_diffuse_pool():
