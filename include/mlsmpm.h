@@ -338,7 +338,9 @@ int mpm_batch_step_times(mpm_batch *b, float *ms, uint32_t cap, uint32_t *count)
  * memory held for mpm_server_result, device memory released).  A request's
  * result is bit-identical to stepping its scene alone.  Served scenes re-sort
  * every step, carry no diffuse pool and keep no i32 grids (sort_period,
- * flags and device of the submitted configuration are set accordingly).
+ * flags and device of the submitted configuration are set accordingly), and
+ * all use the default drift margin: tile_margin 0 or 2 is accepted (and set
+ * to 2), any other value is MPM_ERR_UNSUPPORTED at submit.
  */
 typedef struct mpm_server mpm_server;
 

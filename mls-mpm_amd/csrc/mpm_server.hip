@@ -33,6 +33,9 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Drift margin of every served scene (the members of one batch share it).
+constexpr uint32_t kServedMargin = 2;
+
 struct Request {
   uint64_t id = 0;
   mpm_config cfg{};
@@ -357,6 +360,12 @@ int mpm_server_submit(mpm_server *srv, const mpm_config *cfg, uint32_t n, const 
   c.flags &= ~(MPM_FLAG_NO_LDS | MPM_FLAG_KEEP_GRID);
   if (c.diffuse_capacity)
     return sfail(MPM_ERR_UNSUPPORTED, "served scenes carry no diffuse pool");
+  // every resident scene shares one batch, whose members share one drift
+  // margin: served scenes all take the default
+  if (c.tile_margin && c.tile_margin != kServedMargin)
+    return sfail(MPM_ERR_UNSUPPORTED, "served scenes use tile_margin %u (got %u)", kServedMargin,
+                 c.tile_margin);
+  c.tile_margin = kServedMargin;
   if (n > c.capacity) return sfail(MPM_ERR_CAPACITY, "%u particles > capacity %u", n, c.capacity);
   int rc = validate_config(c);
   if (rc) return rc;

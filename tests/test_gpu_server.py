@@ -141,3 +141,23 @@ def test_invalid_request_fails_cleanly():
         ok = srv.submit(mlsmpm.default_config(*g, len(p)), p, 2, vel=v)
         assert srv.wait(ok, 60.0) == capi.MPM_REQ_DONE
         srv.result(ok)
+
+
+def test_submit_normalises_the_drift_margin():
+    """Every served scene joins one batch, whose members share one drift
+    margin: margin 0 and 2 are accepted and served together, any other is
+    refused at submit, before anything runs."""
+    with mlsmpm.Server(hbm_budget=16 * GB, chunk_steps=4) as srv:
+        g, p, v = scene_inputs(0)
+        with pytest.raises(mlsmpm.MpmError) as e:
+            srv.submit(mlsmpm.default_config(*g, len(p), tile_margin=1), p, 5, vel=v)
+        assert e.value.code == capi.MPM_ERR_UNSUPPORTED
+        ids = []
+        for i, margin in ((0, 2), (2, 0), (4, 2)):
+            g, p, v = scene_inputs(i)
+            cfg = mlsmpm.default_config(*g, len(p), tile_margin=margin)
+            ids.append((srv.submit(cfg, p, 9 + i, vel=v), g, p, v, 9 + i))
+        for rid, g, p, v, k in ids:
+            assert srv.wait(rid, 60.0) == capi.MPM_REQ_DONE
+            assert_same(srv.result(rid), alone(g, p, v, k), f"request {rid}")
+        assert srv.stats()["failed"] == 0
