@@ -112,6 +112,14 @@ int mpm_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, co
  * enqueue nsteps whole steps on the scene's stream. */
 int mpm_step(mpm_scene *s, uint32_t nsteps);
 
+/* No counterpart in the reference (WebGPU compiles its pipelines at
+ * creation): capture the HIP graphs that the next multi-step mpm_step will
+ * replay, without running a step, so that call pays no capture.  Before the
+ * first step after mpm_reset they are the graphs of the state that step
+ * leaves.  A no-op where steps run eagerly (MPM_FLAG_NO_GRAPH, sort period
+ * 0, no particles, batch members). */
+int mpm_prepare(mpm_scene *s);
+
 /* Wait for all work enqueued on the scene's stream. */
 int mpm_sync(mpm_scene *s);
 

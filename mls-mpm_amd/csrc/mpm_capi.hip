@@ -130,6 +130,11 @@ int mpm_step(mpm_scene *s, uint32_t nsteps) {
   return scene_step(s, nsteps);
 }
 
+int mpm_prepare(mpm_scene *s) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_prepare: scene is NULL");
+  return scene_prepare(s);
+}
+
 int mpm_sync(mpm_scene *s) {
   if (!s) return fail(MPM_ERR_INVALID, "mpm_sync: scene is NULL");
   HIPCHK(hipSetDevice(s->cfg.device));

@@ -19,14 +19,16 @@ namespace mpm {
     if (e_ != hipSuccess) return hip_fail(e_, #call);  \
   } while (0)
 
-static char g_msg[512];
+// Formats into a buffer of the calling thread's stack: the server loop and
+// its client threads report errors concurrently.
 static int failf(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 static int failf(int code, const char *fmt, ...) {
+  char msg[512];
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+  vsnprintf(msg, sizeof msg, fmt, ap);
   va_end(ap);
-  return set_error(code, g_msg);
+  return set_error(code, msg);
 }
 
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -781,6 +783,36 @@ int scene_step(mpm_scene *s, uint32_t nsteps) {
     nsteps--;
   }
   return MPM_OK;
+}
+
+// Capture, without running anything, the graphs that the next multi-step
+// call will replay: those of the current start state, or, before the first
+// step after a reset (which sorts the caller's order eagerly), those of the
+// state that step leaves (buffers swapped, one step since the sort).
+int scene_prepare(mpm_scene *s) {
+  const uint32_t period = s->cfg.sort_period;
+  const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS) && period != 0;
+  const uint32_t L = graph_steps(period);
+  if (s->batch || !L || (s->cfg.flags & MPM_FLAG_NO_GRAPH) || !tiled || !s->n ||
+      (s->timing && 2 * L > (uint32_t)mpm_scene::kEvSteps))
+    return MPM_OK;
+  HIPCHK(hipSetDevice(s->cfg.device));
+  const int cur0 = s->cur;
+  const uint32_t since0 = s->since_sort;
+  const bool sorted0 = s->sorted_once;
+  if (!s->sorted_once) {
+    s->cur ^= 1;
+    s->since_sort = 1;
+    s->sorted_once = true;
+  }
+  int rc = MPM_OK;
+  if (s->timing) rc = fold_timing(s);
+  mpm_scene::GraphSet *gs = nullptr;
+  if (!rc) rc = find_graphs(s, &gs);
+  s->cur = cur0;
+  s->since_sort = since0;
+  s->sorted_once = sorted0;
+  return rc;
 }
 
 int scene_read_particles(mpm_scene *s, float *pos4, float *vel4, float *C9, float *density) {

@@ -17,7 +17,7 @@
 // over the per-brick counts gives brick offsets, the work items of the new
 // order and the inverse permutation; G2P then writes every particle's new
 // state straight into its new slot, so the re-sort moves no extra data.  Only
-// the first step after a reset (caller order) runs a separate gather pass.
+// the first step after a reset (caller order) runs a separate scatter pass.
 //
 // P2G.  An item's block owns an LDS window: its brick, the 1-cell stencil
 // halo and a drift margin of M cells ((10+2M)^3 cells).  The block adds the
@@ -96,45 +96,98 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve_update(Params P, GS
     resolve_update_entry<kResolveCells>(P, g, bw, W, keep, bw.list[i]);
 }
 
-// First-step keys: brick and rank of every particle.  Neighbouring slots
-// mostly share a brick (a reset fills bricks row by row), so the wavefront
-// takes one global atomic per distinct brick among its lanes and hands out
-// the ranks by popcount, instead of 64 atomics on one counter.
+// First-step keys: brick and rank of every particle.  A block takes 4096
+// consecutive slots (16 per thread, wavefronts over consecutive slots).  A
+// reset fills bricks row by row, so a block holds large runs of a few dozen
+// bricks: its particles are counted per brick in an LDS table (open
+// addressing), one LDS atomic per brick per wavefront, and the block then
+// reserves each brick's range with one global atomic.  Ranks inside a
+// wavefront's group follow lane order, so consecutive ranks stay consecutive
+// source slots.  A brick that finds no table entry within kProbe probes
+// (more distinct bricks than the table holds) takes the global counter
+// directly, one atomic per wavefront group.
+constexpr uint32_t kKeysPerThread = 16, kKeyTable = 1024, kProbe = 32;
+constexpr uint32_t kNoEntry = 0xffffffffu;
 __global__ __launch_bounds__(256) void k_sort_keys(Params P, PState S, BrickWork bw, KeyOut ko) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = s < P.n;
-  uint32_t dest = 0u;
-  if (live) {
-    int bx, by, bz;
-    cell_brick(P, S.at(PX, s), S.at(PY, s), S.at(PZ, s), bx, by, bz);
-    dest = bw.brick_rank[brick_index(bw, bx, by, bz)];
-    ko.keys[s] = dest;
+  __shared__ uint32_t tkey[kKeyTable], tcnt[kKeyTable];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < kKeyTable; i += 256u) {
+    tkey[i] = kNoEntry;
+    tcnt[i] = 0u;
   }
-  // every lane stays in the loop: the shuffles read all 64 lanes
+  __syncthreads();
   const uint32_t lane = __lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
-  uint64_t pending = __ballot(live);
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t ld = __shfl(dest, leader, 64);
-    const bool mine = live && dest == ld && ((pending >> lane) & 1ull);
-    const uint64_t same = __ballot(mine);
-    uint32_t base = 0u;
-    if (lane == (uint32_t)leader) base = atomicAdd(&ko.counts[ld], (uint32_t)__popcll(same));
-    base = __shfl(base, leader, 64);
-    if (mine) ko.rank[s] = base + (uint32_t)__popcll(same & below);
-    pending &= ~same;
+  const uint32_t base = blockIdx.x * (256u * kKeysPerThread);
+  uint32_t ent[kKeysPerThread], pre[kKeysPerThread];
+#pragma unroll
+  for (uint32_t j = 0; j < kKeysPerThread; j++) {
+    const uint32_t s = base + j * 256u + t;
+    const bool live = s < P.n;
+    uint32_t dest = 0u, e = kNoEntry;
+    if (live) {
+      int bx, by, bz;
+      cell_brick(P, S.at(PX, s), S.at(PY, s), S.at(PZ, s), bx, by, bz);
+      dest = bw.brick_rank[brick_index(bw, bx, by, bz)];
+      ko.keys[s] = dest;
+      uint32_t h = (dest * 2654435761u) >> 22;
+      for (uint32_t k = 0; k < kProbe; k++) {
+        const uint32_t prev = atomicCAS(&tkey[h], kNoEntry, dest);
+        if (prev == kNoEntry || prev == dest) {
+          e = h;
+          break;
+        }
+        h = (h + 1u) & (kKeyTable - 1u);
+      }
+    }
+    // group the lanes by table entry (or by brick, without one); the group's
+    // first lane counts it with one atomic
+    uint64_t pending = __ballot(live);
+    uint32_t leader_of = lane, prefix = 0u, group = 0u;
+    while (pending) {
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const uint32_t le = __shfl(e, leader, 64), ld = __shfl(dest, leader, 64);
+      const bool mine = live && e == le && dest == ld && ((pending >> lane) & 1ull);
+      const uint64_t same = __ballot(mine);
+      if (mine) {
+        leader_of = (uint32_t)leader;
+        prefix = (uint32_t)__popcll(same & below);
+      }
+      if (lane == (uint32_t)leader) group = (uint32_t)__popcll(same);
+      pending &= ~same;
+    }
+    uint32_t gb = 0u;
+    if (group) gb = e != kNoEntry ? atomicAdd(&tcnt[e], group) : atomicAdd(&ko.counts[dest], group);
+    gb = __shfl(gb, (int)leader_of, 64);
+    ent[j] = e;
+    pre[j] = gb + prefix;  // rank in the block's share (table) or final rank (global)
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < kKeyTable; i += 256u)
+    if (tkey[i] != kNoEntry) tcnt[i] = atomicAdd(&ko.counts[tkey[i]], tcnt[i]);
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < kKeysPerThread; j++) {
+    const uint32_t s = base + j * 256u + t;
+    if (s < P.n) ko.rank[s] = ent[j] != kNoEntry ? tcnt[ent[j]] + pre[j] : pre[j];
   }
 }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
-// Gather form of the first step's data move: slot d takes old slot src_of[d].
-__global__ __launch_bounds__(256) void k_sort_gather(uint32_t n, PState src, PState dst,
-                                                     const uint32_t *src_of) {
-  uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= n) return;
-  uint32_t s = src_of[d];
+// The first step's data move: every particle to slot offsets[brick] + rank.
+// The order inside a brick's range is free (every pass reads an item's slots
+// as a set), so this keeps the order the keys pass handed out: a reset fills
+// bricks row by row, consecutive source slots mostly take consecutive ranks,
+// and the loads and the stores both stay contiguous.  (A block-shaped form,
+// 16 slots per thread as in the keys pass, measured slower: 227 against
+// 187 us on the 4.2M-particle dam break, at 4 blocks per CU.)
+__global__ __launch_bounds__(256) void k_sort_scatter(uint32_t n, PState src, PState dst,
+                                                      const uint32_t *keys, const uint32_t *rank,
+                                                      const uint32_t *offsets) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t d = offsets[keys[s]] + rank[s];
 #pragma unroll
   for (int k = 0; k < NPLANES; k++) dst.at(k, d) = src.at(k, s);
   dst.id[d] = src.id[s];
@@ -225,13 +278,13 @@ static void offsets_items_src(const Params &P, const SortBufs &sb, const BrickWo
 }
 
 // Full sort for the first step after a reset: brick keys, offsets, work items
-// and a gather of every particle into its slot.  Leaves the counts zeroed.
+// and a scatter of every particle into its slot.  Leaves the counts zeroed.
 void launch_sort(const Params &P, const PState &src, const PState &dst, const SortBufs &sb,
                  const BrickWork &bw, const TileCfg &tc, hipStream_t st) {
   if (!P.n) return;
-  k_sort_keys<<<nblk(P.n, 256), 256, 0, st>>>(P, src, bw, key_out(sb, bw));
-  offsets_items_src(P, sb, bw, tc, st);
-  k_sort_gather<<<nblk(P.n, 256), 256, 0, st>>>(P.n, src, dst, sb.src_of);
+  k_sort_keys<<<nblk(P.n, 256 * kKeysPerThread), 256, 0, st>>>(P, src, bw, key_out(sb, bw));
+  k_scan_items<<<1, 1024, 0, st>>>(bw, log2_pow2(kTB * tc.ppt), sb.counts, sb.offsets);
+  k_sort_scatter<<<nblk(P.n, 256), 256, 0, st>>>(P.n, src, dst, sb.keys, sb.rank, sb.offsets);
 }
 
 // After a P2G mass pass that recorded keys: offsets, the next step's work

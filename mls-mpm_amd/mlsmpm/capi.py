@@ -146,6 +146,7 @@ _SIGNATURES = {
     "mpm_reset": ([_VP, C.c_uint32, _VP, _VP, _VP, _VP], C.c_int),
     "mpm_step": ([_VP, C.c_uint32], C.c_int),
     "mpm_sync": ([_VP], C.c_int),
+    "mpm_prepare": ([_VP], C.c_int),
     "mpm_get_counts": ([_VP, _VP, _VP], C.c_int),
     "mpm_read_particles": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
     "mpm_read_grid": ([_VP, _VP, _VP, _VP, _VP], C.c_int),

@@ -72,6 +72,10 @@ class Scene:
     def step(self, nsteps: int = 1):
         check(load().mpm_step(self._h, nsteps))
 
+    def prepare(self):
+        """Capture the graphs the next multi-step call replays (mpm_prepare)."""
+        check(load().mpm_prepare(self._h))
+
     def sync(self):
         check(load().mpm_sync(self._h))
 
