@@ -259,6 +259,18 @@ int mpm_delete_near(mpm_scene *s, const float origin[3], const float dir[3], flo
 int mpm_raycast(mpm_scene *s, const float origin[3], const float dir[3], float radius,
                 float *t, int *hit);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight frame or serving loops, harnesses that forbid hipMalloc):
+ * mpm_raycast for nrays rays at once (origins3, dirs3: nrays * 3 floats,
+ * radii: nrays), t[i] and hit[i] exactly as mpm_raycast gives for ray i.  It
+ * calls no allocator: its scratch is a library-static device array of
+ * MPM_RAYCAST_NOALLOC_MAX rays (36 KiB on each GPU it runs on, held until
+ * the process ends), so nrays above that is MPM_ERR_INVALID.  Calls that
+ * overlap on one GPU share that scratch and race.  Synchronous. */
+#define MPM_RAYCAST_NOALLOC_MAX 1024
+int mpm_raycast_noalloc(mpm_scene *s, uint32_t nrays, const float *origins3, const float *dirs3,
+                        const float *radii, float *t, int *hit);
+
 /* Replaces the upload of FluidGPU.spawnParticles (fluid-gpu.js:145-180): n
  * particles (positions inside the hard bounds; vel3 may be NULL) appended
  * after the current ones with unit mass and C = 0, ids continuing the reset

@@ -237,6 +237,12 @@ int mpm_raycast(mpm_scene *s, const float origin[3], const float dir[3], float r
   return scene_raycast(s, origin, dir, radius, t, hit);
 }
 
+int mpm_raycast_noalloc(mpm_scene *s, uint32_t nrays, const float *origins3, const float *dirs3,
+                        const float *radii, float *t, int *hit) {
+  if (!s) return fail(MPM_ERR_INVALID, "mpm_raycast_noalloc: scene is NULL");
+  return scene_raycast_noalloc(s, nrays, origins3, dirs3, radii, t, hit);
+}
+
 int mpm_append_particles(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3) {
   if (!s) return fail(MPM_ERR_INVALID, "mpm_append_particles: scene is NULL");
   return scene_append(s, n, pos3, vel3);

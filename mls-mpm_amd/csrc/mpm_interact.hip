@@ -2,7 +2,14 @@
 // applied inside the step, deleteNear's compaction, raycastFluid and the
 // upload of spawnParticles (fluid-gpu.js:145-180, 479-638; shaders.js:80-84,
 // 394-433, 521-577).
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
 #include "mpm_kernels.h"
+#include "mpm_scene.h"
 
 namespace mpm {
 
@@ -80,6 +87,10 @@ __global__ __launch_bounds__(256) void k_del_move(PState S, PState D, uint32_t n
   D.id[d] = d;
 }
 
+// Ray count of the allocation-free raycast: its scratch holds this many
+// rays and hit words.
+constexpr uint32_t kRaycastMaxRays = MPM_RAYCAST_NOALLOC_MAX;
+
 // WGSL_RAYCAST (shaders.js:562-576): atomic minimum of t's bits (positive
 // floats order like their bit patterns).
 __global__ __launch_bounds__(256) void k_raycast(PState S, uint32_t n, Ray r, uint32_t *hit) {
@@ -96,6 +107,38 @@ __global__ __launch_bounds__(256) void k_raycast(PState S, uint32_t n, Ray r, ui
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Many rays in one pass over the particles, for the allocation-free entry
+// point: every thread tests its particle against each ray with the same
+// arithmetic as k_raycast; a wavefront folds its lanes' hits of a ray with
+// shuffles before one atomic minimum.  rays and hit live in the library's
+// static scratch.
+__global__ __launch_bounds__(256) void k_raycast_many(PState S, uint32_t n, const Ray *rays,
+                                                      uint32_t nrays, uint32_t *hit) {
+  __shared__ Ray lr[kRaycastMaxRays];
+  for (uint32_t i = threadIdx.x; i < nrays; i += blockDim.x) lr[i] = rays[i];
+  __syncthreads();
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = s < n;
+  F3 p = f3(0.0f, 0.0f, 0.0f);
+  if (live) p = f3(S.at(PX, s), S.at(PY, s), S.at(PZ, s));
+  for (uint32_t i = 0; i < nrays; i++) {
+    const Ray r = lr[i];
+    uint32_t cand = 0xffffffffu;
+    if (live) {
+      const F3 to = f3(p.x - r.ox, p.y - r.oy, p.z - r.oz);
+      const float t = dot3(to, f3(r.dx, r.dy, r.dz));
+      if (t > 0.0f && ray_perp_dist(p, r) < r.radius) cand = __float_as_uint(t);
+    }
+    if (!__ballot(cand != 0xffffffffu)) continue;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint32_t o = __shfl_xor(cand, off, 64);
+      cand = o < cand ? o : cand;
+    }
+    if (__lane_id() == 0) atomicMin(&hit[i], cand);
+  }
 }
 
 static inline uint32_t nblocks(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
@@ -117,6 +160,95 @@ void launch_delete_mark_move(const PState &S, const PState &D, uint32_t n, const
 void launch_raycast(const PState &S, uint32_t n, const Ray &r, uint32_t *hit, hipStream_t st) {
   k_fill_u32<<<1, 1, 0, st>>>(hit, 1, 0xffffffffu);
   if (n) k_raycast<<<nblocks(n, 256), 256, 0, st>>>(S, n, r, hit);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free raycast.  Its scratch is this file's static device array:
+// the rays (32 bytes each) and their hit words, one copy per GPU, held until
+// the process ends.  Host code reaches a device's copy only through the
+// address hipGetSymbolAddress returns on that device, looked up once per
+// device at the first call there.  Every call on one GPU shares that copy:
+// calls that overlap on one GPU race.
+constexpr size_t kRaycastScratchBytes = (size_t)kRaycastMaxRays * (sizeof(Ray) + 4);
+static __device__ __align__(256) unsigned char scratch[kRaycastScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+static int scratch_of_device(int dev, void **out) {
+  if (dev < 0 || dev >= 64) return set_error(MPM_ERR_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(raycast scratch)");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MPM_OK;
+}
+
+int scene_raycast_noalloc(mpm_scene *s, uint32_t nrays, const float *origins3, const float *dirs3,
+                          const float *radii, float *t_out, int *hit_out) {
+  if (nrays > kRaycastMaxRays) {
+    char msg[160];
+    snprintf(msg, sizeof msg, "mpm_raycast_noalloc: %u rays, at most %u per call", nrays,
+             kRaycastMaxRays);
+    return set_error(MPM_ERR_INVALID, msg);
+  }
+  if (!nrays) return MPM_OK;
+  if (!origins3 || !dirs3 || !radii || !t_out || !hit_out)
+    return set_error(MPM_ERR_INVALID, "mpm_raycast_noalloc: NULL argument");
+  Ray rays[kRaycastMaxRays];
+  for (uint32_t i = 0; i < nrays; i++) {
+    for (int a = 0; a < 3; a++)
+      if (!std::isfinite(origins3[3 * i + a]) || !std::isfinite(dirs3[3 * i + a]))
+        return set_error(MPM_ERR_INVALID, "mpm_raycast_noalloc: origin and dir must be finite");
+    if (!(radii[i] >= 0.0f) || !std::isfinite(radii[i]))
+      return set_error(MPM_ERR_INVALID, "mpm_raycast_noalloc: radius must be finite and >= 0");
+    Ray &r = rays[i];
+    r.ox = origins3[3 * i];
+    r.oy = origins3[3 * i + 1];
+    r.oz = origins3[3 * i + 2];
+    r.dx = dirs3[3 * i];
+    r.dy = dirs3[3 * i + 1];
+    r.dz = dirs3[3 * i + 2];
+    r.radius = radii[i];
+    r.pad = 0.0f;
+  }
+  hipError_t e = hipSetDevice(s->cfg.device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  void *base = nullptr;
+  int rc = scratch_of_device(s->cfg.device, &base);
+  if (rc) return rc;
+  Ray *drays = (Ray *)base;
+  uint32_t *dhit = (uint32_t *)((char *)base + (size_t)kRaycastMaxRays * sizeof(Ray));
+  PState S;
+  S.pl = s->planes[s->cur];
+  S.id = s->ids[s->cur];
+  S.density = s->dens[s->cur];
+  S.cap = s->cfg.capacity ? s->cfg.capacity : 1;
+  const uint32_t n = s->n;
+  uint32_t bits[kRaycastMaxRays];
+  e = hipMemcpyAsync(drays, rays, (size_t)nrays * sizeof(Ray), hipMemcpyHostToDevice, s->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(rays)");
+  // the previous call's hits stay in the scratch: clear every word used
+  k_fill_u32<<<nblocks(nrays, 256), 256, 0, s->stream>>>(dhit, nrays, 0xffffffffu);
+  if (n) k_raycast_many<<<nblocks(n, 256), 256, 0, s->stream>>>(S, n, drays, nrays, dhit);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "k_raycast_many");
+  e = hipMemcpyAsync(bits, dhit, (size_t)nrays * 4, hipMemcpyDeviceToHost, s->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(hits)");
+  e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  for (uint32_t i = 0; i < nrays; i++) {
+    const int h = bits[i] != 0xffffffffu;
+    hit_out[i] = h;
+    float f = 0.0f;
+    if (h) std::memcpy(&f, &bits[i], 4);
+    t_out[i] = f;
+  }
+  return MPM_OK;
 }
 
 }  // namespace mpm

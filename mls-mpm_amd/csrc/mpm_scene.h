@@ -127,6 +127,8 @@ int scene_delete_near(mpm_scene *s, const float *origin, const float *dir, float
                       uint32_t *remaining);
 int scene_raycast(mpm_scene *s, const float *origin, const float *dir, float radius, float *t,
                   int *hit);
+int scene_raycast_noalloc(mpm_scene *s, uint32_t nrays, const float *origins3, const float *dirs3,
+                          const float *radii, float *t, int *hit);
 int scene_append(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3);
 int scene_export_device(mpm_scene *s, const mpm_export *dst, uint32_t flags);
 int scene_frame_begin(mpm_scene *s, uint32_t what, uint64_t *ticket);

@@ -21,6 +21,7 @@ MPM_ERR_UNSUPPORTED = -6
 MPM_ERR_STATE = -7
 MPM_ERR_NOT_FOUND = -8
 
+MPM_RAYCAST_NOALLOC_MAX = 1024
 MPM_EXPORT_POS = 1
 MPM_EXPORT_VEL = 2
 MPM_EXPORT_DENSITY = 4
@@ -160,6 +161,7 @@ _SIGNATURES = {
     "mpm_set_hand": ([_VP, _VP], C.c_int),
     "mpm_delete_near": ([_VP, _VP, _VP, C.c_float, _VP], C.c_int),
     "mpm_raycast": ([_VP, _VP, _VP, C.c_float, _VP, _VP], C.c_int),
+    "mpm_raycast_noalloc": ([_VP, C.c_uint32, _VP, _VP, _VP, _VP, _VP], C.c_int),
     "mpm_append_particles": ([_VP, C.c_uint32, _VP, _VP], C.c_int),
     "mpm_export_device": ([_VP, _VP, C.c_uint32], C.c_int),
     "mpm_frame_begin": ([_VP, C.c_uint32, _VP], C.c_int),

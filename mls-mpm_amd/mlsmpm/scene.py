@@ -194,6 +194,18 @@ class Scene:
         check(load().mpm_raycast(self._h, _p(o), _p(d), float(radius), C.byref(t), C.byref(hit)))
         return bool(hit.value), float(t.value)
 
+    def raycast_noalloc(self, origins, directions, radii):
+        """mpm_raycast for many rays in one call, through the library's
+        static scratch (no device allocation): (hit bool array, t array)."""
+        o = np.ascontiguousarray(np.asarray(origins, np.float32).reshape(-1, 3))
+        d = np.ascontiguousarray(np.asarray(directions, np.float32).reshape(-1, 3))
+        k = o.shape[0]
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(radii, np.float32), (k,)))
+        t = np.zeros(k, np.float32)
+        hit = np.zeros(k, np.int32)
+        check(load().mpm_raycast_noalloc(self._h, k, _p(o), _p(d), _p(r), _p(t), _p(hit)))
+        return hit.astype(bool), t
+
     def append(self, pos, vel=None):
         """spawnParticles' upload: new particles after the current ones, unit
         mass, zero affine momentum."""
