@@ -236,6 +236,7 @@ def serve_bench(args) -> dict:
     wall = time.perf_counter() - t0
     st = srv.stats()
     step_ms = srv.step_times()
+    rec = srv.step_records()
     srv.close()
     # plain form: a seeded sample of the requests, each scene alone running
     # its request's steps from the initial state (after a warm-up that
@@ -273,7 +274,38 @@ def serve_bench(args) -> dict:
         "request_latency_s": {"p50": float(np.percentile(lat, 50)),
                               "p99": float(np.percentile(lat, 99))},
         "batched_steps": st["batched_steps"],
+        "chunks": st["chunks"],
+        # where the loop thread's time went, and how much of the busy time the
+        # device spent in batched steps
+        "loop_seconds": {k: st[k + "_seconds"] for k in ("enqueue", "admit", "wait", "retire",
+                                                         "enroll")},
+        "gpu_step_seconds": st["gpu_step_seconds"],
+        "gpu_busy_frac": st["gpu_step_seconds"] / max(st["busy_seconds"], 1e-9),
+        "tail": tail_attribution(rec),
     }
+
+
+def tail_attribution(rec) -> dict:
+    """What the slowest batched steps had resident: the steps at or above the
+    99th percentile of latency against the rest (resident particles and
+    scenes, admissions running beside them), and the latency per million
+    resident particles, whose spread is what the mix itself does not explain."""
+    if len(rec) == 0:
+        return {}
+    ms = rec["ms"].astype(np.float64)
+    part = rec["resident_particles"].astype(np.float64)
+    slow = ms >= np.percentile(ms, 99)
+    per_m = ms / np.maximum(part / 1e6, 1e-9)
+
+    def summ(m):
+        return {"steps": int(m.sum()), "ms_mean": float(ms[m].mean()),
+                "resident_particles_mean": float(part[m].mean()),
+                "resident_scenes_mean": float(rec["resident"][m].mean()),
+                "admitting_frac": float((rec["admitting"][m] > 0).mean())}
+
+    return {"p99_steps": summ(slow), "other_steps": summ(~slow),
+            "ms_per_million_particles": {"p50": float(np.percentile(per_m, 50)),
+                                         "p99": float(np.percentile(per_m, 99))}}
 
 
 def handoff_bench(s, n: int, steps_ms: float) -> dict:
@@ -419,7 +451,7 @@ def main() -> int:
     stage = np.zeros(len(STAGES), np.float32)
     step_ms = np.zeros(0, np.float32)
     events_wall = None
-    g_samples = []
+    g_samples, a_samples = [], []
     if not args.no_events:
         s.reset(pos, vel)
         s.enable_timing(True)
@@ -435,6 +467,8 @@ def main() -> int:
             t_ev += time.perf_counter() - t2
             done += k
             g_samples.append(int(np.count_nonzero(s.read_grid(ints=False)["cell_mass_f"])))
+            if D:
+                a_samples.append(s.diffuse_alive())
         events_wall = t_ev / args.steps * 1e3
         stage = s.stage_times()
         step_ms = s.step_times()
@@ -468,9 +502,10 @@ def main() -> int:
             valu = {"floor_ms": ve["floor_ms"], "frac": ve["floor_ms"] / ms[dom],
                     "insts_per_launch": ve["insts_valu"]}
     d_alive = s.diffuse_alive() if D else 0
+    d_alive_mean = float(np.mean(a_samples)) if a_samples else float(d_alive)
     # diffuse term: density read per particle, alive flag per slot, 72 B per
-    # alive record (spawn and kill traffic left out)
-    step_alg = 156 * n + 76 * g_act + (4 * n + 4 * D + 72 * d_alive if D else 0)
+    # alive record at the run's mean alive count (spawn and kill traffic left out)
+    step_alg = 156 * n + 76 * g_act + (4 * n + 4 * D + 72 * d_alive_mean if D else 0)
     step_ms_mean = elapsed / args.steps * 1e3
     line = {
         "metric": "particle-steps/s + % HBM roofline; batched/served vs plain; p99 step latency",
@@ -493,6 +528,7 @@ def main() -> int:
             "sort_period": args.sort_period,
             "diffuse": D,
             "diffuse_alive_end": d_alive,
+            "diffuse_alive_mean": d_alive_mean,
             "lifetime_mode": args.lifetime_mode,
             "active_cells_mean": g_act,
             "parallelism": "single GPU",

@@ -390,6 +390,15 @@ typedef struct mpm_server_stats {
     uint64_t resident_steps;  /* sum over batched steps of the resident scenes */
     double busy_seconds;      /* loop time spent stepping resident scenes */
     double elapsed_seconds;   /* since mpm_server_create */
+    /* Where the loop's time went (seconds, loop thread): */
+    double enqueue_seconds;   /* enqueuing batched chunks (with descriptor table rebuilds) */
+    double admit_seconds;     /* creating, loading and first-stepping admitted scenes */
+    double wait_seconds;      /* waiting for a chunk to finish on the device */
+    double retire_seconds;    /* reading back and releasing finished requests */
+    double enroll_seconds;    /* joining prepared scenes to the batch */
+    double gpu_step_seconds;  /* sum of the batched steps' device times (HIP events) */
+    uint64_t chunks;          /* loop iterations that stepped the batch */
+    uint64_t admitted;        /* requests admitted */
 } mpm_server_stats;
 
 typedef struct mpm_request_info {
@@ -421,6 +430,17 @@ int mpm_server_cancel(mpm_server *srv, uint64_t id);
 int mpm_server_get_stats(mpm_server *srv, mpm_server_stats *out);
 /* GPU time of every batched step so far (HIP events), in milliseconds. */
 int mpm_server_step_times(mpm_server *srv, float *ms, uint32_t cap, uint32_t *count);
+
+/* The same steps with what was resident: for attributing the latency tail. */
+typedef struct mpm_step_record {
+    float ms;                    /* device time of the batched step */
+    uint32_t resident;           /* scenes stepped */
+    uint32_t admitting;          /* scenes being created, loaded and first-stepped meanwhile */
+    uint32_t pad;
+    uint64_t resident_particles; /* particles stepped */
+} mpm_step_record;
+int mpm_server_step_records(mpm_server *srv, mpm_step_record *rec, uint32_t cap,
+                            uint32_t *count);
 
 #ifdef __cplusplus
 }

@@ -11,12 +11,12 @@ struct DiffuseBufs {
   int32_t *top;        // stack top
   uint32_t *seed;      // last-used DiffuseGPU._seed
   uint32_t *genbits;   // spawner bitmask by reset-order id, ceil(cap/32) words
-  uint32_t *gensrc;    // slot of each spawner, by id (cap words)
+  float4 *genval;      // per spawner id: (slot bits, score, speed, density), cap entries
   uint32_t *killbits;  // killed-slot bitmask, ceil(dmax/32) words
   uint32_t *alivebits; // alive flag of every slot (a copy of the records'), ceil(dmax/32) words
-  uint32_t *prefix;    // scan output, max(ceil(cap/32), ceil(dmax/32)) words
+  uint32_t *list;      // dmax words: the spawners in rank order
   uint32_t *scratch;   // scan tile sums
-  uint32_t *total;     // [0] scan total, [1] stack top after the spawns
+  uint32_t *total;     // [0] spawners, [1] stack top after the spawns
 };
 
 // RayParams (shaders.js:78): origin, cylinder radius, direction.
@@ -55,8 +55,9 @@ void launch_g2p_relayout(const Params &P, const PState &S, const PState &D, cons
                          const Walls &W, const uint32_t *src_of, hipStream_t st);
 // Rebuilds the alive bitmask from the pool records (after a host write).
 void launch_alive_bits(const DiffuseBufs &D, uint32_t dmax, hipStream_t st);
+// decided: the G2P pass already marked the spawners (GenOut).
 void launch_diffuse(const Params &P, const PState &S, const GState &g, const Walls &W,
-                    const DiffuseBufs &D, hipStream_t st);
+                    const DiffuseBufs &D, hipStream_t st, bool decided = false);
 uint32_t scan_scratch_words(uint32_t n);
 void launch_scan(const uint32_t *in, uint32_t n, bool popc, uint32_t *out, uint32_t *scratch,
                  uint32_t *total, hipStream_t st, uint32_t *clear = nullptr);

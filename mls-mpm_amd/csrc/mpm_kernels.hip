@@ -145,101 +145,56 @@ __global__ __launch_bounds__(256) void k_g2p_relayout(Params P, PState S, PState
 // one of the orders the reference allows: a decide pass marks spawners in a
 // bitmask indexed by reset-order id, a scan ranks them, and an apply pass
 // gives the r-th spawner free_list[top-1-r] while r < top.
-struct GenEval {
-  float score, speed, density;
-  uint32_t rand_seed;
-};
-
 __device__ __forceinline__ bool gen_eval(const Params &P, const PState &S, const GState &g,
                                          uint32_t s, uint32_t seed, GenEval &e) {
-  float density = S.density[s];
+  const float density = S.density[s];
   if (density <= 0.0f || density > P.density_bubble_min) return false;
-  float vx = S.at(VX, s), vy = S.at(VY, s), vz = S.at(VZ, s);
-  float speed2 = (vx * vx + vy * vy) + vz * vz;
-  float speed = sqrtf(speed2);
-  float m = P.unit_mass ? 1.0f : S.at(PM, s);
-  float ke = (0.5f * m) * speed2;
-  float pressure = eos_pressure(P, density);
-  float trappedAir = fmaxf(0.0f, pressure - P.pressure_threshold);
-
-  // estimate_normal (shaders.js:766-787) on the decoded cell mass.
-  float px = S.at(PX, s), py = S.at(PY, s), pz = S.at(PZ, s);
-  int gxi = (int)floorf(px), gyi = (int)floorf(py), gzi = (int)floorf(pz);
-  int X = (int)P.gx, Y = (int)P.gy, Z = (int)P.gz;
-  float waveCrest = 0.0f;
-  if (!(gxi < 1 || gxi >= X - 1 || gyi < 1 || gyi >= Y - 1 || gzi < 1 || gzi >= Z - 1)) {
-    uint32_t c = (uint32_t)((gzi * Y + gyi) * X + gxi);
-    float mL = g.v[c - 1].w, mR = g.v[c + 1].w;
-    float mD = g.v[c - X].w, mU = g.v[c + X].w;
-    float mB = g.v[c - X * Y].w, mF = g.v[c + X * Y].w;
-    F3 n = f3(mL - mR, mD - mU, mB - mF);
-    float len = sqrtf(dot3(n, n));
-    if (len >= 1e-6f && speed > 1e-4f) {
-      F3 nn = f3(n.x / len, n.y / len, n.z / len);
-      float dotVal = ((vx / speed) * nn.x + (vy / speed) * nn.y) + (vz / speed) * nn.z;
-      if (dotVal > P.crest_dot_threshold) waveCrest = dotVal * speed;
-    }
-  }
-  float score = (ke + trappedAir * 2.0f) + waveCrest * 1.5f;
-  if (score < P.ke_threshold) return false;
-  uint32_t randSeed = S.id[s] * 9781u + seed;
-  float spawnChance = fminf(1.0f, ((score - P.ke_threshold) * P.spawn_rate_k) * P.dt);
-  if (diffuse_rand(randSeed) > spawnChance) return false;
-  e.score = score;
-  e.speed = speed;
-  e.density = density;
-  e.rand_seed = randSeed;
-  return true;
+  return gen_decide(P, g, density, S.at(VX, s), S.at(VY, s), S.at(VZ, s), S.at(PX, s),
+                    S.at(PY, s), S.at(PZ, s), P.unit_mass ? 1.0f : S.at(PM, s), S.id[s], seed, e);
 }
 
+// Marks the spawners by id and keeps what the apply pass needs of each: its
+// slot and the score, speed and density it was decided on.
 __global__ __launch_bounds__(256) void k_gen_decide(Params P, PState S, GState g,
                                                     const uint32_t *seedp, uint32_t *bits,
-                                                    uint32_t *src) {
+                                                    float4 *val) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
   GenEval e;
   if (!gen_eval(P, S, g, s, *seedp + 1u, e)) return;
-  uint32_t id = S.id[s];
-  src[id] = s;
-  atomicOr(&bits[id >> 5], 1u << (id & 31));
+  gen_mark(GenOut{seedp, bits, val}, S.id[s], s, e);
 }
 
-__global__ __launch_bounds__(256) void k_gen_apply(Params P, PState S, GState g,
-                                                   const uint32_t *seedp, uint32_t *bits,
-                                                   const uint32_t *prefix, const uint32_t *src,
-                                                   const uint32_t *free_list, const int32_t *topp,
-                                                   DRec *pool, uint32_t *alivebits,
-                                                   uint32_t nwords) {
-  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nwords) return;
-  uint32_t word = bits[w];
-  if (!word) return;
-  bits[w] = 0;
-  int32_t top0 = *topp;
-  uint32_t seed = *seedp + 1u;
-  uint32_t r = prefix[w];
-  while (word) {
-    int b = __ffs(word) - 1;
-    word &= word - 1;
-    uint32_t id = w * 32u + (uint32_t)b;
-    int32_t rank = (int32_t)(r++);
-    if (rank >= top0) continue;
-    int32_t fi = top0 - 1 - rank;
+// The r-th spawner in id order (list[r], from the scan) takes the slot
+// free_list[top - 1 - r] while r < top; grid-stride over the spawners.
+__global__ __launch_bounds__(256) void k_gen_apply(Params P, PState S, const uint32_t *seedp,
+                                                   const uint32_t *list, const uint32_t *total,
+                                                   const float4 *val, const uint32_t *free_list,
+                                                   const int32_t *topp, DRec *pool,
+                                                   uint32_t *alivebits) {
+  const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t top0 = *topp;
+  const uint32_t nsp = *total, seed = *seedp + 1u;
+  const uint32_t lim = top0 > 0 ? (nsp < (uint32_t)top0 ? nsp : (uint32_t)top0) : 0u;
+  for (uint32_t r = t0; r < lim; r += gridDim.x * blockDim.x) {
+    const int32_t fi = top0 - 1 - (int32_t)r;
     if (fi < 0 || (uint32_t)fi >= P.dmax) continue;  // only on an inconsistent pool
-    uint32_t slot = free_list[fi];
+    const uint32_t slot = free_list[fi];
     if (slot >= P.dmax) continue;
-    uint32_t s = src[id];
-    GenEval e;
-    if (!gen_eval(P, S, g, s, seed, e)) continue;  // same inputs: always true here
+    const uint32_t id = list[r];
+    const float4 v = val[id];
+    const uint32_t s = __float_as_uint(v.x);
+    const float score = v.y, speed = v.z, density = v.w;
+    const uint32_t rand_seed = id * 9781u + seed;
     float jitter = 0.3f;
-    float jx = ((diffuse_rand(e.rand_seed + 1u) - 0.5f) * jitter) * e.speed;
-    float jy = ((diffuse_rand(e.rand_seed + 2u) - 0.5f) * jitter) * e.speed;
-    float jz = ((diffuse_rand(e.rand_seed + 3u) - 0.5f) * jitter) * e.speed;
-    uint32_t ptype = (e.density < P.density_spray_max) ? 0u : 1u;
+    float jx = ((diffuse_rand(rand_seed + 1u) - 0.5f) * jitter) * speed;
+    float jy = ((diffuse_rand(rand_seed + 2u) - 0.5f) * jitter) * speed;
+    float jz = ((diffuse_rand(rand_seed + 3u) - 0.5f) * jitter) * speed;
+    uint32_t ptype = (density < P.density_spray_max) ? 0u : 1u;
     float lifetime = 999.0f;
     if (ptype == 1u)
-      lifetime = P.lifetime_mode ? 0.5f + 2.5f * diffuse_rand(e.rand_seed + 4u)
-                                 : P.lifetime_foam * fminf(1.0f, e.score / P.ke_threshold);
+      lifetime = P.lifetime_mode ? 0.5f + 2.5f * diffuse_rand(rand_seed + 4u)
+                                 : P.lifetime_foam * fminf(1.0f, score / P.ke_threshold);
     DRec rec;
     rec.pos = make_float4(S.at(PX, s), S.at(PY, s), S.at(PZ, s), 0.0f);
     rec.vel = make_float4(S.at(VX, s) + jx, S.at(VY, s) + jy, S.at(VZ, s) + jz, 0.0f);
@@ -252,6 +207,7 @@ __global__ __launch_bounds__(256) void k_gen_apply(Params P, PState S, GState g,
   }
 }
 
+// No particles: no spawners.
 __global__ void k_zero_total(uint32_t *p) { *p = 0u; }
 
 // top -= min(spawners, top); the step's seed becomes the last-used seed.
@@ -305,12 +261,16 @@ __device__ __forceinline__ void reflect_axis(float &p, float &v, float lo, float
 }
 
 // The alive bitmask mirrors the records' alive flags, so dead slots are
-// skipped without touching the pool (32 slots per word read).
+// skipped without touching the pool (32 slots per word read); a kill clears
+// both.  Thread 0 also closes the spawns (gen_finish): nothing here reads the
+// stack top or the seed.  (Advecting a compacted list of the alive slots
+// instead measured slower: the compaction's atomics plus the list's
+// indirection cost more than the skipped threads.)
 __global__ __launch_bounds__(256) void k_advect(Params P, GState g, DRec *pool, uint32_t *killbits,
                                                 uint32_t *alivebits, Walls W, int32_t *topp,
                                                 const uint32_t *total, uint32_t *seedp,
                                                 int32_t *topsnap) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) gen_finish(topp, total, seedp, topsnap);
   if (i >= P.dmax) return;
   if (!((alivebits[i >> 5] >> (i & 31)) & 1u)) return;
@@ -378,28 +338,6 @@ __global__ __launch_bounds__(256) void k_alive_bits(const DRec *pool, uint32_t *
     if (pool[w * 32u + b].alive != 0u) bits |= 1u << b;
   alivebits[w] = bits;
 }
-
-// Pushes the killed slots in slot order above topsnap (the stack top after
-// the spawns); one thread moves top on by the number killed.
-__global__ __launch_bounds__(256) void k_kill_push(uint32_t *killbits, const uint32_t *prefix,
-                                                   uint32_t *free_list, const int32_t *topsnap,
-                                                   int32_t *topp, const uint32_t *total,
-                                                   uint32_t nwords, uint32_t dmax) {
-  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w == 0) *topp = *topsnap + (int32_t)*total;
-  if (w >= nwords) return;
-  uint32_t word = killbits[w];
-  if (!word) return;
-  killbits[w] = 0;
-  int32_t pos = *topsnap + (int32_t)prefix[w];
-  while (word) {
-    int b = __ffs(word) - 1;
-    word &= word - 1;
-    if (pos >= 0 && (uint32_t)pos < dmax) free_list[pos] = w * 32u + (uint32_t)b;
-    pos++;
-  }
-}
-
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of u32 counts (or of the popcounts of bitmask words):
@@ -488,6 +426,56 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in, 
       if (clear) clear[base + i] = 0u;
     }
     run += v[i];
+  }
+}
+
+// Down pass over a bitmask that writes the index of every set bit at its
+// rank: out[*base + rank] for ranks whose position lies in [0, cap) (base
+// null: 0), and clears the bitmask behind it.  Block 0 stores the number of
+// set bits in *total (if set) and *base + that number in *top (if set).
+// Spawners are listed for the apply pass this way (ranks by id), and the
+// killed slots are pushed onto the free list above the top (in slot order).
+__global__ __launch_bounds__(kScanThreads) void k_scan_emit(uint32_t *bits, uint32_t n,
+                                                           const uint32_t *partials, uint32_t nb,
+                                                           uint32_t *out, const int32_t *base,
+                                                           uint32_t cap, uint32_t *total,
+                                                           int32_t *top) {
+  uint32_t before = 0, all = 0;
+  for (uint32_t i = threadIdx.x; i < nb; i += kScanThreads) {
+    const uint32_t p = partials[i];
+    all += p;
+    if (i < blockIdx.x) before += p;
+  }
+  uint32_t unused;
+  before = block_exclusive_scan(before, unused);
+  all = block_exclusive_scan(all, unused);
+  const int32_t b0 = base ? *base : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (total) *total = all;
+    if (top) *top = b0 + (int32_t)all;
+  }
+  const uint32_t wbase = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    v[i] = wbase + i < n ? bits[wbase + i] : 0u;
+    sum += (uint32_t)__popc(v[i]);
+  }
+  uint32_t excl;
+  block_exclusive_scan(sum, excl);
+  int64_t pos = (int64_t)b0 + before + excl;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    uint32_t word = v[i];
+    if (!word) continue;
+    bits[wbase + i] = 0u;
+    while (word) {
+      const int bit = __ffs(word) - 1;
+      word &= word - 1;
+      if (pos >= 0 && pos < (int64_t)cap) out[pos] = (wbase + i) * 32u + (uint32_t)bit;
+      pos++;
+    }
   }
 }
 
@@ -598,25 +586,33 @@ void launch_alive_bits(const DiffuseBufs &D, uint32_t dmax, hipStream_t st) {
   if (dmax) k_alive_bits<<<blocks((dmax + 31u) / 32u, 256), 256, 0, st>>>(D.pool, D.alivebits, dmax);
 }
 
+// The diffuse step: decide (spawners by id; or done by G2P) -> scan that
+// lists them in id order -> apply (one thread per spawner) -> advect over
+// the alive slots (closes the spawns) -> scan that pushes the killed slots.
 void launch_diffuse(const Params &P, const PState &S, const GState &g, const Walls &W,
-                    const DiffuseBufs &D, hipStream_t st) {
+                    const DiffuseBufs &D, hipStream_t st, bool decided) {
   if (!P.dmax) return;
-  uint32_t nwg = blocks(P.n, 32);
+  int32_t *topsnap = (int32_t *)(D.total + 1);
   if (P.n) {
-    k_gen_decide<<<blocks(P.n, 256), 256, 0, st>>>(P, S, g, D.seed, D.genbits, D.gensrc);
-    launch_scan(D.genbits, nwg, true, D.prefix, D.scratch, D.total, st);
-    k_gen_apply<<<blocks(nwg, 256), 256, 0, st>>>(P, S, g, D.seed, D.genbits, D.prefix, D.gensrc,
-                                                  D.free_list, D.top, D.pool, D.alivebits, nwg);
+    const uint32_t nwg = blocks(P.n, 32);
+    if (!decided)
+      k_gen_decide<<<blocks(P.n, 256), 256, 0, st>>>(P, S, g, D.seed, D.genbits, D.genval);
+    const uint32_t nb = blocks(nwg, kScanTile);
+    k_scan_reduce<true><<<nb, kScanThreads, 0, st>>>(D.genbits, nwg, D.scratch);
+    k_scan_emit<<<nb, kScanThreads, 0, st>>>(D.genbits, nwg, D.scratch, nb, D.list, nullptr,
+                                             P.dmax, D.total, nullptr);
+    const uint32_t ab = blocks(P.dmax, 256);
+    k_gen_apply<<<ab < 1024 ? ab : 1024, 256, 0, st>>>(P, S, D.seed, D.list, D.total, D.genval,
+                                                       D.free_list, D.top, D.pool, D.alivebits);
   } else {
     k_zero_total<<<1, 1, 0, st>>>(D.total);
   }
-  int32_t *topsnap = (int32_t *)(D.total + 1);
   k_advect<<<blocks(P.dmax, 256), 256, 0, st>>>(P, g, D.pool, D.killbits, D.alivebits, W, D.top,
                                                D.total, D.seed, topsnap);
-  uint32_t nwk = blocks(P.dmax, 32);
-  launch_scan(D.killbits, nwk, true, D.prefix, D.scratch, D.total, st);
-  k_kill_push<<<blocks(nwk, 256), 256, 0, st>>>(D.killbits, D.prefix, D.free_list, topsnap, D.top,
-                                               D.total, nwk, P.dmax);
+  const uint32_t nwk = blocks(P.dmax, 32), kb = blocks(nwk, kScanTile);
+  k_scan_reduce<true><<<kb, kScanThreads, 0, st>>>(D.killbits, nwk, D.scratch);
+  k_scan_emit<<<kb, kScanThreads, 0, st>>>(D.killbits, nwk, D.scratch, kb, D.free_list, topsnap,
+                                           P.dmax, nullptr, D.top);
 }
 
 void launch_export_particles(const PState &S, uint32_t n, float4 *pos4, float4 *vel4, float *C9,

@@ -263,6 +263,71 @@ __device__ __forceinline__ void g2p_particle(const Params &P, const Walls &W, fl
   o.vel = make_float3(nv.x, nv.y, nv.z);
 }
 
+// Spawn decision of WGSL_DIFFUSE_GENERATE (shaders.js:805-829) for one fluid
+// particle: kinetic energy, trapped air from the EOS pressure, the wave
+// crest from the 6-neighbour decoded-mass gradient (estimate_normal,
+// shaders.js:766-787, read from the velocity grid's mass lane), the score
+// and the seeded spawn chance.  The reference pops free-list slots with
+// unordered atomics (try_spawn, shaders.js:791-798); the callers here mark
+// the spawners in a bitmask by reset-order id instead, which a scan ranks.
+struct GenEval {
+  float score, speed, density;
+  uint32_t rand_seed;
+};
+
+__device__ __forceinline__ bool gen_decide(const Params &P, const GState &g, float density,
+                                           float vx, float vy, float vz, float px, float py,
+                                           float pz, float m, uint32_t id, uint32_t seed,
+                                           GenEval &e) {
+  if (density <= 0.0f || density > P.density_bubble_min) return false;
+  float speed2 = (vx * vx + vy * vy) + vz * vz;
+  float speed = sqrtf(speed2);
+  float ke = (0.5f * m) * speed2;
+  float pressure = eos_pressure(P, density);
+  float trappedAir = fmaxf(0.0f, pressure - P.pressure_threshold);
+  int gxi = (int)floorf(px), gyi = (int)floorf(py), gzi = (int)floorf(pz);
+  int X = (int)P.gx, Y = (int)P.gy, Z = (int)P.gz;
+  float waveCrest = 0.0f;
+  if (!(gxi < 1 || gxi >= X - 1 || gyi < 1 || gyi >= Y - 1 || gzi < 1 || gzi >= Z - 1)) {
+    uint32_t c = (uint32_t)((gzi * Y + gyi) * X + gxi);
+    float mL = g.v[c - 1].w, mR = g.v[c + 1].w;
+    float mD = g.v[c - X].w, mU = g.v[c + X].w;
+    float mB = g.v[c - X * Y].w, mF = g.v[c + X * Y].w;
+    F3 n = f3(mL - mR, mD - mU, mB - mF);
+    float len = sqrtf(dot3(n, n));
+    if (len >= 1e-6f && speed > 1e-4f) {
+      F3 nn = f3(n.x / len, n.y / len, n.z / len);
+      float dotVal = ((vx / speed) * nn.x + (vy / speed) * nn.y) + (vz / speed) * nn.z;
+      if (dotVal > P.crest_dot_threshold) waveCrest = dotVal * speed;
+    }
+  }
+  float score = (ke + trappedAir * 2.0f) + waveCrest * 1.5f;
+  if (score < P.ke_threshold) return false;
+  uint32_t randSeed = id * 9781u + seed;
+  float spawnChance = fminf(1.0f, ((score - P.ke_threshold) * P.spawn_rate_k) * P.dt);
+  if (diffuse_rand(randSeed) > spawnChance) return false;
+  e.score = score;
+  e.speed = speed;
+  e.density = density;
+  e.rand_seed = randSeed;
+  return true;
+}
+
+// Where a G2P pass that also decides the diffuse spawners marks them: the
+// spawner bitmask by id and each spawner's (slot bits, score, speed,
+// density); bits == null: no decision.
+struct GenOut {
+  const uint32_t *seed;  // the last-used seed; this step's generate uses seed + 1
+  uint32_t *bits;
+  float4 *val;
+};
+
+__device__ __forceinline__ void gen_mark(const GenOut &go, uint32_t id, uint32_t slot,
+                                         const GenEval &e) {
+  go.val[id] = make_float4(__uint_as_float(slot), e.score, e.speed, e.density);
+  atomicOr(&go.bits[id >> 5], 1u << (id & 31));
+}
+
 // Store a G2P result into slot d of a particle state.
 __device__ __forceinline__ void store_g2p(const PState &S, uint32_t d, const G2POut &o) {
 #pragma unroll

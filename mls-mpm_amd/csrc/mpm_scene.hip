@@ -89,7 +89,7 @@ size_t scene_footprint(const mpm_config &c) {
   b += nit * (16 + 4 * 4000);            // work items and ghost cores
   b += nb * 8 + 65536;                   // flags, lists, scan scratch
   b += cap * 72;                         // read-back staging
-  if (D) b += D * (48 + 4) + cap * 4 + (cap / 32 + 2 * (D / 32) + 3) * 8 + 4096;
+  if (D) b += D * (48 + 4 + 4) + cap * 16 + (cap / 32 + 2 * (D / 32) + 3) * 8 + 4096;
   return b;
 }
 
@@ -342,12 +342,12 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     DALLOC(s->d.top, 4);
     DALLOC(s->d.seed, 4);
     DALLOC(s->d.genbits, (size_t)nwg * 4);
-    DALLOC(s->d.gensrc, cap * 4);
+    DALLOC(s->d.genval, cap * 16);
     DALLOC(s->d.killbits, (size_t)nwk * 4);
     DALLOC(s->d.alivebits, (size_t)nwk * 4);
-    DALLOC(s->d.prefix, (size_t)nw * 4);
+    DALLOC(s->d.list, (size_t)D * 4);
     DALLOC(s->d.scratch, (size_t)scan_scratch_words(nw) * 4);
-    DALLOC(s->d.total, 8);  // scan total; the stack top after the spawns
+    DALLOC(s->d.total, 8);  // spawners; the stack top after the spawns
     HIPCHK(hipMemsetAsync(s->d.genbits, 0, (size_t)nwg * 4, s->stream));
     HIPCHK(hipMemsetAsync(s->d.killbits, 0, (size_t)nwk * 4, s->stream));
   }
@@ -601,21 +601,23 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   if ((rc = mark(4))) return rc;
   if (relayout) HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
   if ((rc = mark(5))) return rc;
+  // with a diffuse pool, the tiled G2P also decides the spawners
+  const GenOut go = (tiled && P.dmax) ? GenOut{s->d.seed, s->d.genbits, s->d.genval} : GenOut{};
   if (relayout) {
     PState dst = pstate(s, s->cur ^ 1);
-    launch_g2p_tiled(P, S, dst, s->g, s->walls, s->tc, bw, s->sort.src_of, st);
+    launch_g2p_tiled(P, S, dst, s->g, s->walls, s->tc, bw, s->sort.src_of, st, go);
     s->cur ^= 1;
     S = dst;
     s->since_sort = 0;
   } else if (tiled) {
-    launch_g2p_tiled(P, S, S, s->g, s->walls, s->tc, bw, nullptr, st);
+    launch_g2p_tiled(P, S, S, s->g, s->walls, s->tc, bw, nullptr, st, go);
   } else {
     launch_g2p_global(P, S, s->g, s->walls, st);
   }
   if ((rc = mark(6))) return rc;
   // the resolve list starts every step empty (the re-sorting G2P empties it)
   if (tiled && !relayout) launch_clear_list(bw, st);
-  launch_diffuse(P, S, s->g, s->walls, s->d, st);
+  launch_diffuse(P, S, s->g, s->walls, s->d, st, go.bits != nullptr);
   if ((rc = mark(7))) return rc;
   HIPCHK(hipGetLastError());
   s->steps++;

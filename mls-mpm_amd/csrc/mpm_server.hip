@@ -76,6 +76,7 @@ struct mpm_server {
   uint64_t bytes_resident = 0;
   mpm_server_stats stats{};
   std::vector<float> step_ms;                          // per batched step
+  std::vector<mpm_step_record> step_rec;               // per batched step
   std::thread loop;
   mpm_batch *batch = nullptr;
   int loop_error = 0;
@@ -232,26 +233,44 @@ void run_loop(mpm_server *srv) {
     for (Request *r : srv->active) c = std::min(c, r->steps - r->done);
     const double tb = srv->now();
     int rc = stepping ? mpm_batch_step(srv->batch, c) : MPM_OK;
+    const double t_enq = srv->now();
     std::vector<Request *> prepared;
     for (Request *r : admitting)
       if (prepare(srv, r) == MPM_OK) prepared.push_back(r);
+    const double t_prep = srv->now();
     auto join_prepared = [&]() {
+      const double t0 = srv->now();
       for (Request *r : prepared) {
         if (r->done >= r->steps) retire(srv, r, false);
         else enroll(srv, r);
       }
+      std::lock_guard<std::mutex> lk(srv->mu);
+      srv->stats.enroll_seconds += srv->now() - t0;
     };
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      srv->stats.enqueue_seconds += t_enq - tb;
+      srv->stats.admit_seconds += t_prep - t_enq;
+      srv->stats.admitted += admitting.size();
+    }
     if (!stepping) {
       join_prepared();
       continue;
     }
     if (!rc) rc = mpm_batch_sync(srv->batch);
+    const double t_sync = srv->now();
     uint32_t cnt = 0;
     if (!rc) rc = mpm_batch_step_times(srv->batch, nullptr, 0, &cnt);
     if (!rc && cnt) {
       ms.resize(cnt);
       rc = mpm_batch_step_times(srv->batch, ms.data(), cnt, &cnt);
       mpm_batch_enable_timing(srv->batch, 1);  // restart the record
+    }
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      srv->stats.wait_seconds += t_sync - t_prep;
+      srv->stats.chunks++;
+      for (uint32_t i = 0; i < cnt && i < ms.size(); i++) srv->stats.gpu_step_seconds += ms[i] * 1e-3;
     }
     if (rc) {
       std::string msg = mpm_last_error();
@@ -293,9 +312,24 @@ void run_loop(mpm_server *srv) {
       srv->stats.resident_steps += (uint64_t)srv->active.size() * c;
       srv->stats.busy_seconds += srv->now() - tb;
       srv->step_ms.insert(srv->step_ms.end(), ms.begin(), ms.begin() + std::min<size_t>(cnt, ms.size()));
+      uint64_t np = 0;
+      for (Request *r : srv->active) np += r->n;
+      for (uint32_t i = 0; i < cnt && i < ms.size(); i++) {
+        mpm_step_record q{};
+        q.ms = ms[i];
+        q.resident = (uint32_t)srv->active.size();
+        q.admitting = (uint32_t)admitting.size();
+        q.resident_particles = np;
+        srv->step_rec.push_back(q);
+      }
     }
+    const double t_ret = srv->now();
     for (Request *r : finished) retire(srv, r, false);
     for (Request *r : cancelled) retire(srv, r, true);
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      srv->stats.retire_seconds += srv->now() - t_ret;
+    }
     if (!finished.empty() || !cancelled.empty()) {
       auto keep = [&](Request *r) {
         return std::find(finished.begin(), finished.end(), r) == finished.end() &&
@@ -477,6 +511,17 @@ int mpm_server_get_stats(mpm_server *srv, mpm_server_stats *out) {
   out->active = (uint32_t)srv->active.size();
   out->bytes_resident = srv->bytes_resident;
   out->elapsed_seconds = srv->now();
+  return MPM_OK;
+}
+
+int mpm_server_step_records(mpm_server *srv, mpm_step_record *rec, uint32_t cap,
+                            uint32_t *count) {
+  if (!srv) return sfail(MPM_ERR_INVALID, "mpm_server_step_records: server is NULL");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  const uint32_t n = (uint32_t)srv->step_rec.size();
+  if (count) *count = n;
+  if (rec)
+    for (uint32_t i = 0; i < n && i < cap; i++) rec[i] = srv->step_rec[i];
   return MPM_OK;
 }
 

@@ -1,6 +1,7 @@
 // mpm_tiled.h -- launchers of the sorted / LDS-staged kernels (internal).
 #pragma once
 #include "mpm_kernels.h"
+#include "mpm_ops.h"
 
 namespace mpm {
 
@@ -57,9 +58,11 @@ void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, con
                           const BrickWork &bw, hipStream_t st);
 void launch_resolve_update(const Params &P, const GState &g, const TileCfg &tc,
                            const BrickWork &bw, const Walls &W, bool keep, hipStream_t st);
+// go.bits set: the pass also decides the diffuse spawners (launch_diffuse
+// with decided = true skips its own decide pass).
 void launch_g2p_tiled(const Params &P, const PState &S, const PState &D, const GState &g,
                       const Walls &W, const TileCfg &tc, const BrickWork &bw,
-                      const uint32_t *src_of, hipStream_t st);
+                      const uint32_t *src_of, hipStream_t st, const GenOut &go = GenOut{});
 void launch_sort(const Params &P, const PState &src, const PState &dst, const SortBufs &sb,
                  const BrickWork &bw, const TileCfg &tc, hipStream_t st);
 void launch_relayout_prep(const Params &P, const SortBufs &sb, const BrickWork &bw,

@@ -62,11 +62,11 @@ __global__ __launch_bounds__(64) void k_brick_list(BrickWork bw) {
 }
 
 
-template <int M, bool EXACT>
+template <int M, bool EXACT, bool GEN>
 __global__ __launch_bounds__(256) void k_g2p_brick(Params P, PState S, PState D, GState g,
-                                                   Walls W, BrickWork bw,
-                                                   const uint32_t *src_of) {
-  g2p_item<M, EXACT>(P, S, D, g, W, bw, src_of, blockIdx.x);
+                                                   Walls W, BrickWork bw, const uint32_t *src_of,
+                                                   GenOut go) {
+  g2p_item<M, EXACT, GEN>(P, S, D, g, W, bw, src_of, blockIdx.x, go);
 }
 
 // Offsets and work items of the new order, one block.
@@ -253,12 +253,18 @@ void launch_resolve_update(const Params &P, const GState &g, const TileCfg &tc,
 // order, exact windows.  In place: margin as configured.
 void launch_g2p_tiled(const Params &P, const PState &S, const PState &D, const GState &g,
                       const Walls &W, const TileCfg &tc, const BrickWork &bw,
-                      const uint32_t *src_of, hipStream_t st) {
+                      const uint32_t *src_of, hipStream_t st, const GenOut &go) {
   if (!P.n) return;
   uint32_t grid = max_items(P.n, bw.nbricks, tc.ppt), tb = tc.g2p_threads;
-  if (src_of) k_g2p_brick<0, true><<<grid, tb, 0, st>>>(P, S, D, g, W, bw, src_of);
-  else if (tc.margin == 1) k_g2p_brick<1, false><<<grid, tb, 0, st>>>(P, S, D, g, W, bw, nullptr);
-  else k_g2p_brick<2, false><<<grid, tb, 0, st>>>(P, S, D, g, W, bw, nullptr);
+#define G2P_LAUNCH(M_, EX_, SRC_)                                                             \
+  do {                                                                                        \
+    if (go.bits) k_g2p_brick<M_, EX_, true><<<grid, tb, 0, st>>>(P, S, D, g, W, bw, SRC_, go);  \
+    else k_g2p_brick<M_, EX_, false><<<grid, tb, 0, st>>>(P, S, D, g, W, bw, SRC_, go);         \
+  } while (0)
+  if (src_of) G2P_LAUNCH(0, true, src_of);
+  else if (tc.margin == 1) G2P_LAUNCH(1, false, nullptr);
+  else G2P_LAUNCH(2, false, nullptr);
+#undef G2P_LAUNCH
 }
 
 KeyOut key_out(const SortBufs &sb, const BrickWork &bw) {

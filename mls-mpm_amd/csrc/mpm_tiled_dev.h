@@ -567,10 +567,16 @@ __device__ __forceinline__ void resolve_update_entry(const Params &P, const GSta
 // (re-sort; id, density and a non-unit mass move along) and every particle lies in the
 // item's brick (EXACT: no particle can leave the window).  Otherwise G2P runs
 // in place over the current items with the drift margin.
-template <int M, bool EXACT>
+//
+// GEN: the pass also takes the diffuse spawn decision of every particle it
+// writes, on its new position and velocity, its density and the grid's
+// decoded mass (what the separate decide pass would read after it), marking
+// the spawners in go.
+template <int M, bool EXACT, bool GEN = false>
 __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const PState &D,
                                          const GState &g, const Walls &W, const BrickWork &bw,
-                                         const uint32_t *src_of, uint32_t blk) {
+                                         const uint32_t *src_of, uint32_t blk,
+                                         const GenOut &go = GenOut{}) {
   constexpr int E = 10 + 2 * M, V = E * E * E;
   __shared__ float4 lv[V];
   // Re-sort: the item's source slots, staged with the window.
@@ -613,6 +619,7 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
   uint32_t idx = t;
   if (idx >= it.z) return;
   const uint32_t gx = P.gx, slice = P.gx * P.gy;
+  const uint32_t gen_seed = GEN ? *go.seed + 1u : 0u;
   // Software pipeline: the next particle's inputs (for a re-sort also the
   // mass, id and density it moves) load while this one computes.  Prefetch
   // indexes past the item's end are clamped to its last particle, so every
@@ -661,6 +668,17 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
       if (!P.unit_mass) D.at(PM, d) = cur.m;
       D.idr(d) = cur.id;
       D.dens(d) = cur.dens;
+    }
+    if (GEN) {
+      const float dens = EXACT ? cur.dens : S.dens(cur.s);
+      const float m = P.unit_mass ? 1.0f : (EXACT ? cur.m : S.at(PM, cur.s));
+      if (dens > 0.0f && dens <= P.density_bubble_min) {
+        const uint32_t id = EXACT ? cur.id : S.idr(cur.s);
+        GenEval e;
+        if (gen_decide(P, g, dens, o.vel.x, o.vel.y, o.vel.z, o.pos.x, o.pos.y, o.pos.z, m, id,
+                       gen_seed, e))
+          gen_mark(go, id, d, e);
+      }
     }
   };
   In cur, nxt;

@@ -103,7 +103,17 @@ class ServerStats(C.Structure):
                 ("bytes_resident", C.c_uint64), ("particle_steps", C.c_uint64),
                 ("batched_steps", C.c_uint64), ("resident_steps", C.c_uint64),
                 ("busy_seconds", C.c_double),
-                ("elapsed_seconds", C.c_double)]
+                ("elapsed_seconds", C.c_double),
+                ("enqueue_seconds", C.c_double), ("admit_seconds", C.c_double),
+                ("wait_seconds", C.c_double), ("retire_seconds", C.c_double),
+                ("enroll_seconds", C.c_double), ("gpu_step_seconds", C.c_double),
+                ("chunks", C.c_uint64), ("admitted", C.c_uint64)]
+
+
+class StepRecord(C.Structure):
+    """mpm_step_record."""
+    _fields_ = [("ms", C.c_float), ("resident", C.c_uint32), ("admitting", C.c_uint32),
+                ("pad", C.c_uint32), ("resident_particles", C.c_uint64)]
 
 
 class RequestInfo(C.Structure):
@@ -191,6 +201,7 @@ _SIGNATURES = {
     "mpm_server_cancel": ([_VP, C.c_uint64], C.c_int),
     "mpm_server_get_stats": ([_VP, _VP], C.c_int),
     "mpm_server_step_times": ([_VP, _VP, C.c_uint32, _VP], C.c_int),
+    "mpm_server_step_records": ([_VP, _VP, C.c_uint32, _VP], C.c_int),
 }
 
 _lib = None

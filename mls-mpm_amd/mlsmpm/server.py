@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 
 from .capi import (Config, MPM_REQ_CANCELLED, MPM_REQ_DONE, MPM_REQ_FAILED, RequestInfo,
-                   ServerConfig, ServerStats, check, load)
+                   ServerConfig, ServerStats, StepRecord, check, load)
 
 _FINAL = (MPM_REQ_DONE, MPM_REQ_FAILED, MPM_REQ_CANCELLED)
 
@@ -89,3 +89,14 @@ class Server:
         ms = np.zeros(max(cnt.value, 1), np.float32)
         check(load().mpm_server_step_times(self._h, _p(ms), cnt.value, C.byref(cnt)))
         return ms[: cnt.value]
+
+    def step_records(self) -> np.ndarray:
+        """Per batched step: device ms, resident scenes, scenes being admitted
+        meanwhile, resident particles."""
+        cnt = C.c_uint32()
+        check(load().mpm_server_step_records(self._h, None, 0, C.byref(cnt)))
+        rec = (StepRecord * max(cnt.value, 1))()
+        check(load().mpm_server_step_records(self._h, rec, cnt.value, C.byref(cnt)))
+        dt = np.dtype([("ms", np.float32), ("resident", np.uint32), ("admitting", np.uint32),
+                       ("pad", np.uint32), ("resident_particles", np.uint64)])
+        return np.frombuffer(bytes(rec), dtype=dt)[: cnt.value].copy()

@@ -118,7 +118,7 @@ def test_batch_and_server_fail_loudly_without_device():
 def test_abi_struct_sizes():
     assert C.sizeof(capi.BatchTuning) == 8 * 4
     assert C.sizeof(capi.ServerConfig) == 48
-    assert C.sizeof(capi.ServerStats) == 4 * 8 + 2 * 4 + 4 * 8 + 2 * 8
+    assert C.sizeof(capi.ServerStats) == 4 * 8 + 2 * 4 + 4 * 8 + 2 * 8 + 6 * 8 + 2 * 8
     assert C.sizeof(capi.RequestInfo) == 4 * 4 + 8 + 3 * 8
 
 
@@ -126,7 +126,7 @@ def test_abi_struct_sizes():
 ABI_STRUCTS = {"Config": "mpm_config", "BatchTuning": "mpm_batch_tuning",
                "ServerConfig": "mpm_server_config", "ServerStats": "mpm_server_stats",
                "RequestInfo": "mpm_request_info", "Hand": "mpm_hand", "Export": "mpm_export",
-               "Frame": "mpm_frame"}
+               "Frame": "mpm_frame", "StepRecord": "mpm_step_record"}
 
 
 def test_ctypes_structs_match_the_header(tmp_path):

@@ -52,7 +52,13 @@ def test_server_results_equal_scenes_stepped_alone():
         st = srv.stats()
         assert st["completed"] == 5 and st["failed"] == 0 and st["active"] == 0
         assert st["bytes_resident"] == 0
-        assert len(srv.step_times()) > 0
+        ms = srv.step_times()
+        assert len(ms) > 0
+        rec = srv.step_records()
+        assert len(rec) == len(ms) and np.array_equal(rec["ms"], ms)
+        assert np.all(rec["resident"] >= 1) and np.all(rec["resident_particles"] >= 4000)
+        assert st["chunks"] > 0 and st["admitted"] == 5
+        assert 0 < st["gpu_step_seconds"] <= st["busy_seconds"] * 1.5
 
 
 def test_same_request_alone_and_in_a_changing_mix():
