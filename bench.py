@@ -127,22 +127,49 @@ def batch_bench(args) -> dict:
     `batch_scenes` dam-break scenes (grid edges log-uniform 32..128 per axis,
     10k..2M particles log-uniform, capped at 8 per cell over half the
     interior), every scene re-sorting every step.  Batched: all scenes in one
-    mpm_batch (one persistent launch per stage, captured graphs, launch shapes
-    from the autotuner).  Sequential: each scene stepped alone (its own
-    captured graphs), one scene after another.  Both forms run the same
-    `batch_steps` steps from the same initial state (a warm-up captures the
-    graphs, then every scene is reset); rates are particle-steps/s."""
+    mpm_batch (one persistent launch per stage, captured graphs, work-item
+    sizes from the shipped autotuner table, persistent block counts tuned on
+    the batch).  Sequential: each scene stepped alone (its own captured
+    graphs), one scene after another, once with the single-scene default
+    work-item size and once with the sizes the autotuner chose for scenes
+    alone.  Every form runs the same `batch_steps` steps from the same
+    initial state (a warm-up captures the graphs, then every scene is reset);
+    rates are particle-steps/s."""
     import mlsmpm
     from mlsmpm import autotune, scenes
 
     mix = scenes.scene_mix(args.seed + 3, args.batch_scenes)
     ntot = sum(len(p) for _, p, _, _ in mix)
     print(f"[bench] batch: {len(mix)} scenes, {ntot} particles", file=sys.stderr, flush=True)
-    sc = []
-    for grid, pos, vel, _ in mix:
-        s = mlsmpm.Scene(mlsmpm.default_config(*grid, len(pos), sort_period=1, device=0))
-        s.reset(pos, vel)
-        sc.append(s)
+    # work-item sizes from the shipped autotuner table: per particle-count
+    # class, tuned for batched steps (the batch) and for scenes alone
+    tb_batch = autotune.load_table(key="batch_ppt_by_class")
+    tb_alone = autotune.load_table(key="alone_ppt_by_class")
+
+    def make(table):
+        out = []
+        for grid, pos, vel, _ in mix:
+            cfg = autotune.apply_tiles(
+                mlsmpm.default_config(*grid, len(pos), sort_period=1, device=0), table)
+            s = mlsmpm.Scene(cfg)
+            s.reset(pos, vel)
+            out.append(s)
+        return out
+
+    def sequential(scs):
+        for s in scs:
+            s.step(W)
+        for s in scs:
+            s.sync()
+        for s, (_, pos, vel, _) in zip(scs, mix):
+            s.reset(pos, vel)
+        t0 = time.perf_counter()
+        for s in scs:
+            s.step(K)
+            s.sync()
+        return time.perf_counter() - t0
+
+    sc = make(tb_batch)
 
     def reset_all():
         for s, (_, pos, vel, _) in zip(sc, mix):
@@ -165,21 +192,21 @@ def batch_bench(args) -> dict:
     b.step(min(K, 32))
     lat = b.step_times()
     b.close()
-    reset_all()
     for s in sc:
-        s.step(W)
+        s.close()
+    # sequential plain steps: the single-scene default work-item size, and the
+    # sizes tuned for scenes alone
+    sc = make(None)
+    tseq = sequential(sc)
     for s in sc:
-        s.sync()
-    reset_all()
-    t0 = time.perf_counter()
-    for s in sc:
-        s.step(K)
-        s.sync()
-    tseq = time.perf_counter() - t0
+        s.close()
+    sc = make(tb_alone)
+    tseq_t = sequential(sc)
     for s in sc:
         s.close()
     batched = ntot * K / tb
     seq = ntot * K / tseq
+    seq_t = ntot * K / tseq_t
     return {
         "scenes": len(mix),
         "particles": ntot,
@@ -190,6 +217,9 @@ def batch_bench(args) -> dict:
         "sequential_plain": seq,
         "sequential_ms_per_step": tseq / K * 1e3,
         "vs_sequential": batched / seq,
+        "sequential_plain_tuned": seq_t,
+        "vs_sequential_tuned": batched / seq_t,
+        "tile_ppt_by_class": {"batched": tb_batch, "sequential_tuned": tb_alone},
         "step_latency_ms": {"p50": float(np.percentile(lat, 50)),
                             "p99": float(np.percentile(lat, 99))},
         "tuning": {"item_blocks": tune.item_blocks, "resolve_blocks": tune.resolve_blocks,
@@ -219,7 +249,12 @@ def serve_bench(args) -> dict:
     work = sum(len(p) * int(k) for (_, p, _, _), k in zip(mix, steps))
     print(f"[bench] serve: {nreq} requests, {work:.3e} particle-steps", file=sys.stderr,
           flush=True)
-    srv = mlsmpm.Server(hbm_budget=200 << 30, chunk_steps=args.serve_chunk)
+    from mlsmpm import autotune
+
+    tb_batch = autotune.load_table(key="batch_ppt_by_class")
+    tb_alone = autotune.load_table(key="alone_ppt_by_class")
+    srv = mlsmpm.Server(hbm_budget=200 << 30, chunk_steps=args.serve_chunk,
+                        tile_ppt_by_class=tb_batch)
     ids = []
     t0 = time.perf_counter()
     for (grid, pos, vel, _), k, ta in zip(mix, steps, arrivals):
@@ -242,19 +277,27 @@ def serve_bench(args) -> dict:
     # its request's steps from the initial state (after a warm-up that
     # captures its graphs and a reset)
     sample = np.random.default_rng(args.seed + 13).choice(nreq, min(nreq, 48), replace=False)
-    plain_s, plain_work = 0.0, 0
-    for i in sample:
-        grid, pos, vel, _ = mix[i]
-        with mlsmpm.Scene(mlsmpm.default_config(*grid, len(pos), sort_period=1)) as s:
-            s.reset(pos, vel)
-            s.step(17)
-            s.reset(pos, vel)
-            ta = time.perf_counter()
-            s.step(int(steps[i]))
-            s.sync()
-            plain_s += time.perf_counter() - ta
-            plain_work += len(pos) * int(steps[i])
-    plain_rate = plain_work / plain_s
+
+    def plain(table):
+        plain_s, plain_work = 0.0, 0
+        for i in sample:
+            grid, pos, vel, _ = mix[i]
+            cfg = autotune.apply_tiles(mlsmpm.default_config(*grid, len(pos), sort_period=1),
+                                       table)
+            with mlsmpm.Scene(cfg) as s:
+                s.reset(pos, vel)
+                s.step(17)
+                s.reset(pos, vel)
+                s.prepare()
+                ta = time.perf_counter()
+                s.step(int(steps[i]))
+                s.sync()
+                plain_s += time.perf_counter() - ta
+                plain_work += len(pos) * int(steps[i])
+        return plain_work / plain_s
+
+    plain_rate = plain(None)
+    plain_rate_t = plain(tb_alone)
     busy_rate = st["particle_steps"] / st["busy_seconds"]
     return {
         "requests": nreq,
@@ -268,6 +311,8 @@ def serve_bench(args) -> dict:
         "plain_rate": plain_rate,
         "plain_sample": int(len(sample)),
         "vs_plain": busy_rate / plain_rate,
+        "plain_rate_tuned": plain_rate_t,
+        "vs_plain_tuned": busy_rate / plain_rate_t,
         "mean_resident": st["resident_steps"] / max(1, st["batched_steps"]),
         "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                             "p99": float(np.percentile(step_ms, 99))},

@@ -378,7 +378,11 @@ typedef struct mpm_server_config {
     uint32_t item_blocks;     /* batch launch shapes (mpm_batch_tuning; 0 = default) */
     uint32_t resolve_blocks;
     uint32_t aux_blocks;
-    uint32_t reserved[4];
+    /* Work-item size (mpm_config.tile_ppt: 4, 8 or 16; 0 = leave it) for a
+     * request of n particles whose own tile_ppt is 0, by particle-count class
+     * floor(log2(n)) / 2, as the autotuner (mlsmpm.autotune) tuned them for
+     * batched steps. */
+    uint8_t tile_ppt_by_class[16];
 } mpm_server_config;
 
 typedef struct mpm_server_stats {
@@ -406,6 +410,8 @@ typedef struct mpm_request_info {
     uint32_t n, steps, steps_done;
     uint64_t footprint;       /* device bytes reserved at admission */
     double t_submit, t_admit, t_done;  /* seconds since mpm_server_create */
+    uint32_t tile_ppt;        /* work-item size the request's scene runs with */
+    uint32_t pad;
 } mpm_request_info;
 
 /* Starts the loop thread. */

@@ -355,6 +355,10 @@ int mpm_server_create(const mpm_server_config *cfg, mpm_server **out) {
   if (!cfg || !out) return sfail(MPM_ERR_INVALID, "mpm_server_create: NULL argument");
   *out = nullptr;
   if (!cfg->hbm_budget) return sfail(MPM_ERR_INVALID, "mpm_server_create: hbm_budget is 0");
+  for (int i = 0; i < 16; i++)
+    if (cfg->tile_ppt_by_class[i] && !tiled_ppt_supported(cfg->tile_ppt_by_class[i]))
+      return sfail(MPM_ERR_INVALID, "mpm_server_create: tile_ppt_by_class[%d] = %u (4, 8 or 16)",
+                   i, (unsigned)cfg->tile_ppt_by_class[i]);
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev == 0)
@@ -400,6 +404,12 @@ int mpm_server_submit(mpm_server *srv, const mpm_config *cfg, uint32_t n, const 
     return sfail(MPM_ERR_UNSUPPORTED, "served scenes use tile_margin %u (got %u)", kServedMargin,
                  c.tile_margin);
   c.tile_margin = kServedMargin;
+  if (!c.tile_ppt && n) {
+    uint32_t lg = 0;
+    while ((n >> (lg + 1)) != 0) lg++;
+    const uint32_t cls = lg / 2, ppt = cls < 16 ? srv->cfg.tile_ppt_by_class[cls] : 0u;
+    if (ppt) c.tile_ppt = ppt;
+  }
   if (n > c.capacity) return sfail(MPM_ERR_CAPACITY, "%u particles > capacity %u", n, c.capacity);
   int rc = validate_config(c);
   if (rc) return rc;
@@ -443,6 +453,8 @@ int mpm_server_poll(mpm_server *srv, uint64_t id, mpm_request_info *info) {
   info->t_admit = r.t_admit;
   info->t_done = r.t_done;
   info->footprint = r.footprint;
+  info->tile_ppt = r.cfg.tile_ppt ? r.cfg.tile_ppt : 8u;
+  info->pad = 0;
   return MPM_OK;
 }
 

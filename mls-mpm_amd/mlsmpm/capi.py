@@ -93,7 +93,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_active", C.c_uint32), ("hbm_budget", C.c_uint64),
                 ("chunk_steps", C.c_uint32), ("item_blocks", C.c_uint32),
                 ("resolve_blocks", C.c_uint32), ("aux_blocks", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("tile_ppt_by_class", C.c_uint8 * 16)]
 
 
 class ServerStats(C.Structure):
@@ -120,7 +120,8 @@ class RequestInfo(C.Structure):
     """mpm_request_info."""
     _fields_ = [("state", C.c_int32), ("n", C.c_uint32), ("steps", C.c_uint32),
                 ("steps_done", C.c_uint32), ("footprint", C.c_uint64), ("t_submit", C.c_double),
-                ("t_admit", C.c_double), ("t_done", C.c_double)]
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("tile_ppt", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
 class Hand(C.Structure):

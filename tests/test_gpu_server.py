@@ -167,3 +167,32 @@ def test_submit_normalises_the_drift_margin():
             assert srv.wait(rid, 60.0) == capi.MPM_REQ_DONE
             assert_same(srv.result(rid), alone(g, p, v, k), f"request {rid}")
         assert srv.stats()["failed"] == 0
+
+
+def test_tile_table_applies_per_class_and_keeps_results():
+    """A per-class work-item table given to the server sets tile_ppt of the
+    requests that leave it 0 (class floor(log2 n) / 2); each request equals
+    its scene stepped alone with that size."""
+    from mlsmpm import autotune
+
+    inputs = [scene_inputs(i) for i in (0, 3, 1)]
+    cls = [autotune.size_class(len(p)) for _, p, _ in inputs]
+    assert len(set(cls[:2])) == 2
+    table = {cls[0]: 4, cls[1]: 16, cls[2]: 16}
+    with mlsmpm.Server(hbm_budget=16 * GB, chunk_steps=4, tile_ppt_by_class=table) as srv:
+        reqs = []
+        for i, want in ((0, 4), (1, 16), (2, 8)):
+            grid, pos, vel = inputs[i]
+            ppt = 8 if i == 2 else 0                # an explicit size is kept
+            cfg = mlsmpm.default_config(*grid, len(pos), tile_ppt=ppt)
+            reqs.append((srv.submit(cfg, pos, 11, vel=vel), grid, pos, vel, want))
+        for rid, grid, pos, vel, want in reqs:
+            assert srv.poll(rid)["tile_ppt"] == want
+            assert srv.wait(rid, 60.0) == capi.MPM_REQ_DONE
+            cfg = mlsmpm.default_config(*grid, len(pos), sort_period=1, tile_ppt=want)
+            with mlsmpm.Scene(cfg) as s:
+                s.reset(pos, vel)
+                s.step(11)
+                assert_same(srv.result(rid), s.read_particles(), f"request {rid}")
+    with pytest.raises(mlsmpm.MpmError):
+        mlsmpm.Server(hbm_budget=GB, tile_ppt_by_class={3: 5})
