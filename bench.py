@@ -87,18 +87,35 @@ def workload(name: str, seed: int):
     return grid, pos, vel, desc
 
 
-def cpu_baseline(grid, pos, vel, seconds: float, max_threads: int = 16,
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(grid, pos, vel, seconds: float, max_threads: int = 0,
                  max_steps: int = 200) -> dict:
     """The oracle (C restatement of the reference kernels, OpenMP) on a
-    bounded number of steps of the same scene, on up to `max_threads` host
-    threads."""
+    bounded number of steps of the same scene, on every host core this
+    process may use: the cores of its affinity mask, capped at the host's
+    share for one GPU when OMP_NUM_THREADS states one (max_threads > 0 caps
+    it further)."""
     from oracle import oracle
 
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    threads = max(1, min(max_threads, cores))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(cores, share) if share > 0 else cores
+    if max_threads > 0:
+        threads = min(threads, max_threads)
+    threads = max(1, threads)
     o = oracle.OracleScene(grid=tuple(grid))
     o.reset(pos, vel)
     t0 = time.perf_counter()
@@ -119,6 +136,9 @@ def cpu_baseline(grid, pos, vel, seconds: float, max_threads: int = 16,
         "kind": "port",
         "sample": f"first {steps} steps of the same scene ({n} particles), "
                   f"{threads} OpenMP threads, {t2 - t0:.1f} s",
+        "affinity_cores": cores,
+        "omp_num_threads": share or None,
+        "cpu_model": cpu_model(),
     }
 
 
@@ -425,8 +445,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16,
-                    help="host threads of the CPU baseline (at most the cores available)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="cap on the CPU baseline's host threads (0: every core available, "
+                         "at most OMP_NUM_THREADS)")
     ap.add_argument("--cpu-max-steps", type=int, default=200)
     ap.add_argument("--no-events", action="store_true",
                     help="skip the event-timed region (no stage times, no roofline)")
@@ -450,14 +471,14 @@ def main() -> int:
         return 2
 
     if args.copy_calibration:
-        import torch
+        from mlsmpm.hipmem import DeviceBuffer
 
         nb = args.copy_calibration
-        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
+        src, dst = DeviceBuffer(nb, fill=1), DeviceBuffer(nb)
         for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+            dst.copy_from(src)
+        src.close()
+        dst.close()
         print(json.dumps({"copy_bytes": nb, "copies": 3}), flush=True)
         return 0
 

@@ -14,7 +14,7 @@ import numpy as np
 from .capi import load
 
 _hip = None
-_D2H, _H2D = 2, 1
+_D2H, _H2D, _D2D = 2, 1, 3
 
 
 def runtime() -> C.CDLL:
@@ -59,6 +59,11 @@ class DeviceBuffer:
         a = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
         _ok(runtime().hipMemcpy(a.ctypes.data, self.ptr, a.nbytes, _D2H), "hipMemcpy")
         return a if shape is None else a.reshape(shape)
+
+    def copy_from(self, src: "DeviceBuffer") -> None:
+        """Device-to-device copy of src's bytes (synchronous)."""
+        n = min(self.nbytes, src.nbytes)
+        _ok(runtime().hipMemcpy(self.ptr, src.ptr, n, _D2D), "hipMemcpy")
 
     def close(self) -> None:
         if self.ptr:
