@@ -13,13 +13,18 @@ namespace mpm {
 constexpr float kFixed = 100000.0f;  // FIXED (shaders.js:15)
 constexpr float kEncMax = 20000.0f;  // encode clamp (shaders.js:21-24)
 
-// Particle state planes (structure of arrays, `cap` floats per plane).
+// Particle state fields.  PX, PY, PZ and the reset-order id of a slot form
+// one 16-byte record (x, y, z, id bits), so that the re-sort's gather of a
+// particle from its old slot is one load and its store one store; the other
+// fields are planes of `cap` floats (structure of arrays).
 enum Plane : int {
   PX = 0, PY, PZ, VX, VY, VZ,
   C00, C01, C02, C10, C11, C12, C20, C21, C22,
   PM,
   NPLANES
 };
+// Floats per slot of the particle state: the 4-float record, then VX .. PM.
+constexpr int kStateFloats = NPLANES + 1;
 
 // Step constants: the Params uniform (shaders.js:27-49) plus the diffuse
 // GenParams / AdvectParams (shaders.js:734-743, 872-877).
@@ -42,18 +47,25 @@ struct Walls { Wall w[4]; };
 
 // Particle state of one scene.
 struct PState {
-  float *pl;         // NPLANES * cap floats
-  uint32_t *id;      // reset-order index of the particle held in each slot
+  float *pl;         // kStateFloats * cap floats: cap (x, y, z, id) records, then VX .. PM
   float *density;    // particle_density, by slot
   uint32_t cap;
   // A slot's byte offset fits in 32 bits (capacity <= 2^26): a uniform
   // plane base plus a 32-bit lane offset, which the compiler can address as
   // a scalar base and one vector offset instead of a 64-bit address per plane.
   __device__ __forceinline__ float &at(int k, uint32_t p) const {
-    return *reinterpret_cast<float *>(reinterpret_cast<char *>(pl + (size_t)k * cap) + (p << 2));
+    if (k < 3)
+      return *reinterpret_cast<float *>(reinterpret_cast<char *>(pl) + (p << 4) + (k << 2));
+    return *reinterpret_cast<float *>(reinterpret_cast<char *>(pl + (size_t)(k + 1) * cap) +
+                                      (p << 2));
   }
+  // The slot's record: position and, in .w, the id's bits.
+  __device__ __forceinline__ float4 &rec(uint32_t p) const {
+    return *reinterpret_cast<float4 *>(reinterpret_cast<char *>(pl) + (p << 4));
+  }
+  // Reset-order index of the particle held in slot p.
   __device__ __forceinline__ uint32_t &idr(uint32_t p) const {
-    return *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(id) + (p << 2));
+    return *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(pl) + (p << 4) + 12);
   }
   __device__ __forceinline__ float &dens(uint32_t p) const {
     return *reinterpret_cast<float *>(reinterpret_cast<char *>(density) + (p << 2));

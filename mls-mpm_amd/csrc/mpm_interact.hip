@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_del_mark(PState S, uint32_t n, Ray r, u
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   if (ray_perp_dist(f3(S.at(PX, s), S.at(PY, s), S.at(PZ, s)), r) < r.radius) return;
-  const uint32_t id = S.id[s];
+  const uint32_t id = S.idr(s);
   atomicOr(&bits[id >> 5], 1u << (id & 31));
 }
 
@@ -78,13 +78,14 @@ __global__ __launch_bounds__(256) void k_del_move(PState S, PState D, uint32_t n
                                                   const uint32_t *bits, const uint32_t *prefix) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const uint32_t id = S.id[s], w = bits[id >> 5], b = id & 31;
+  const uint32_t id = S.idr(s), w = bits[id >> 5], b = id & 31;
   if (!((w >> b) & 1u)) return;
   const uint32_t d = prefix[id >> 5] + (uint32_t)__popc(w & ((1u << b) - 1u));
+  const float4 q = S.rec(s);
+  D.rec(d) = make_float4(q.x, q.y, q.z, __uint_as_float(d));
 #pragma unroll
-  for (int k = 0; k < NPLANES; k++) D.at(k, d) = S.at(k, s);
+  for (int k = VX; k < NPLANES; k++) D.at(k, d) = S.at(k, s);
   D.density[d] = S.density[s];
-  D.id[d] = d;
 }
 
 // Ray count of the allocation-free raycast: its scratch holds this many
@@ -225,7 +226,6 @@ int scene_raycast_noalloc(mpm_scene *s, uint32_t nrays, const float *origins3, c
   uint32_t *dhit = (uint32_t *)((char *)base + (size_t)kRaycastMaxRays * sizeof(Ray));
   PState S;
   S.pl = s->planes[s->cur];
-  S.id = s->ids[s->cur];
   S.density = s->dens[s->cur];
   S.cap = s->cfg.capacity ? s->cfg.capacity : 1;
   const uint32_t n = s->n;

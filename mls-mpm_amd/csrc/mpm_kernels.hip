@@ -99,7 +99,8 @@ __global__ __launch_bounds__(256) void k_update_dense(Params P, GState g, Walls 
 __global__ __launch_bounds__(256) void k_g2p_global(Params P, PState S, GState g, Walls W) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
-  float px = S.at(PX, p), py = S.at(PY, p), pz = S.at(PZ, p);
+  const float4 q = S.rec(p);
+  float px = q.x, py = q.y, pz = q.z;
   PW pw;
   particle_weights(px, py, pz, pw);
   const uint32_t gx = P.gx, slice = P.gx * P.gy;
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) void k_g2p_global(Params P, PState S, GState g
     float4 v = g.v[base + k * slice + j * gx + i];
     return make_float3(v.x, v.y, v.z);
   }, o);
-  store_g2p(S, p, o);
+  store_g2p(S, p, o, __float_as_uint(q.w));
 }
 
 // G2P that also applies the new particle order: destination slot d takes the
@@ -121,7 +122,8 @@ __global__ __launch_bounds__(256) void k_g2p_relayout(Params P, PState S, PState
   uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= P.n) return;
   uint32_t p = src_of[d];
-  float px = S.at(PX, p), py = S.at(PY, p), pz = S.at(PZ, p);
+  const float4 q = S.rec(p);
+  float px = q.x, py = q.y, pz = q.z;
   PW pw;
   particle_weights(px, py, pz, pw);
   const uint32_t gx = P.gx, slice = P.gx * P.gy;
@@ -131,9 +133,8 @@ __global__ __launch_bounds__(256) void k_g2p_relayout(Params P, PState S, PState
     float4 v = g.v[base + k * slice + j * gx + i];
     return make_float3(v.x, v.y, v.z);
   }, o);
-  store_g2p(D, d, o);
+  store_g2p(D, d, o, __float_as_uint(q.w));
   D.at(PM, d) = S.at(PM, p);
-  D.id[d] = S.id[p];
   D.density[d] = S.density[p];
 }
 
@@ -150,7 +151,7 @@ __device__ __forceinline__ bool gen_eval(const Params &P, const PState &S, const
   const float density = S.density[s];
   if (density <= 0.0f || density > P.density_bubble_min) return false;
   return gen_decide(P, g, density, S.at(VX, s), S.at(VY, s), S.at(VZ, s), S.at(PX, s),
-                    S.at(PY, s), S.at(PZ, s), P.unit_mass ? 1.0f : S.at(PM, s), S.id[s], seed, e);
+                    S.at(PY, s), S.at(PZ, s), P.unit_mass ? 1.0f : S.at(PM, s), S.idr(s), seed, e);
 }
 
 // Marks the spawners by id and keeps what the apply pass needs of each: its
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(256) void k_gen_decide(Params P, PState S, GState g
   if (s >= P.n) return;
   GenEval e;
   if (!gen_eval(P, S, g, s, *seedp + 1u, e)) return;
-  gen_mark(GenOut{seedp, bits, val}, S.id[s], s, e);
+  gen_mark(GenOut{seedp, bits, val}, S.idr(s), s, e);
 }
 
 // The r-th spawner in id order (list[r], from the scan) takes the slot
@@ -488,7 +489,7 @@ __global__ __launch_bounds__(256) void k_export_particles(PState S, uint32_t n, 
                                                           float4 *vel4, float *C9, float *dens) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const uint32_t id = RESET ? S.id[s] : s;
+  const uint32_t id = RESET ? S.idr(s) : s;
   if (pos4) pos4[id] = make_float4(S.at(PX, s), S.at(PY, s), S.at(PZ, s), 0.0f);
   if (vel4) vel4[id] = make_float4(S.at(VX, s), S.at(VY, s), S.at(VZ, s), 0.0f);
   if (C9) {
@@ -524,9 +525,8 @@ __global__ __launch_bounds__(256) void k_import_particles(PState S, uint32_t n, 
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint32_t q = first + p;
-  S.at(PX, q) = pos3[3 * (size_t)p];
-  S.at(PY, q) = pos3[3 * (size_t)p + 1];
-  S.at(PZ, q) = pos3[3 * (size_t)p + 2];
+  S.rec(q) = make_float4(pos3[3 * (size_t)p], pos3[3 * (size_t)p + 1], pos3[3 * (size_t)p + 2],
+                         __uint_as_float(q));
   S.at(VX, q) = vel3 ? vel3[3 * (size_t)p] : 0.0f;
   S.at(VY, q) = vel3 ? vel3[3 * (size_t)p + 1] : 0.0f;
   S.at(VZ, q) = vel3 ? vel3[3 * (size_t)p + 2] : 0.0f;
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(256) void k_import_particles(PState S, uint32_t n, 
   for (int k = 0; k < 9; k++) S.at(C00 + k, q) = C9 ? C9[(size_t)p * 9 + k] : 0.0f;
   S.at(PM, q) = mass ? mass[p] : 1.0f;
   S.density[q] = 0.0f;
-  S.id[q] = q;
+
 }
 
 // ---------------------------------------------------------------------------

@@ -18,9 +18,10 @@ struct MomIn {
 };
 template <bool UNIT>
 __device__ __forceinline__ void load_mom_in(const PState &S, uint32_t p, MomIn &in) {
-  in.px = S.at(PX, p);
-  in.py = S.at(PY, p);
-  in.pz = S.at(PZ, p);
+  const float4 q = S.rec(p);
+  in.px = q.x;
+  in.py = q.y;
+  in.pz = q.z;
   in.vx = S.at(VX, p);
   in.vy = S.at(VY, p);
   in.vz = S.at(VZ, p);
@@ -328,13 +329,13 @@ __device__ __forceinline__ void gen_mark(const GenOut &go, uint32_t id, uint32_t
   atomicOr(&go.bits[id >> 5], 1u << (id & 31));
 }
 
-// Store a G2P result into slot d of a particle state.
-__device__ __forceinline__ void store_g2p(const PState &S, uint32_t d, const G2POut &o) {
+// Store a G2P result, with the particle's id, into slot d of a particle
+// state (the position and id as one 16-byte record).
+__device__ __forceinline__ void store_g2p(const PState &S, uint32_t d, const G2POut &o,
+                                          uint32_t id) {
 #pragma unroll
   for (int k = 0; k < 9; k++) S.at(C00 + k, d) = o.C[k];
-  S.at(PX, d) = o.pos.x;
-  S.at(PY, d) = o.pos.y;
-  S.at(PZ, d) = o.pos.z;
+  S.rec(d) = make_float4(o.pos.x, o.pos.y, o.pos.z, __uint_as_float(id));
   S.at(VX, d) = o.vel.x;
   S.at(VY, d) = o.vel.y;
   S.at(VZ, d) = o.vel.z;

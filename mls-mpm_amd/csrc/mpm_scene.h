@@ -16,8 +16,7 @@ struct mpm_scene {
   uint32_t G = 0;        // cells
   uint64_t steps = 0;    // steps since reset
   int cur = 0;           // which particle buffer is current (sorting ping-pongs)
-  float *planes[2] = {nullptr, nullptr};
-  uint32_t *ids[2] = {nullptr, nullptr};
+  float *planes[2] = {nullptr, nullptr};  // (x, y, z, id) records, then the planes VX .. PM
   float *dens[2] = {nullptr, nullptr};  // particle_density, per layout buffer
   int unit_mass = 1;
   mpm::GState g{};

@@ -83,7 +83,7 @@ size_t scene_footprint(const mpm_config &c) {
   const size_t nb = (size_t)cdiv(c.grid_x, 8) * cdiv(c.grid_y, 8) * cdiv(c.grid_z, 8);
   const size_t nit = nb + cap / 1024 + 1;
   size_t b = 0;
-  b += 2 * cap * (NPLANES * 4 + 4 + 4);  // particle planes, ids, density (two orders)
+  b += 2 * cap * (kStateFloats * 4 + 4);  // particle records and planes, density (two orders)
   b += G * (8 * 4 + 16);                 // i32 grids, overflow grids, float4 velocity/mass
   b += cap * 12 + nb * 16;               // keys, ranks, src_of; counts, offsets, first, count
   b += nit * (16 + 4 * 4000);            // work items and ghost cores
@@ -153,7 +153,6 @@ Params scene_params(const mpm_scene *s) {
 static PState pstate(const mpm_scene *s, int which) {
   PState S;
   S.pl = s->planes[which];
-  S.id = s->ids[which];
   S.density = s->dens[which];
   S.cap = s->cfg.capacity ? s->cfg.capacity : 1;
   return S;
@@ -265,8 +264,7 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
   s->G = c.grid_x * c.grid_y * c.grid_z;
   const size_t G = s->G;
   for (int b = 0; b < 2; b++) {
-    DALLOC(s->planes[b], cap * NPLANES * 4);
-    DALLOC(s->ids[b], cap * 4);
+    DALLOC(s->planes[b], cap * kStateFloats * 4);
   }
   DALLOC(s->dens[0], cap * 4);
   DALLOC(s->dens[1], cap * 4);

@@ -188,9 +188,9 @@ __global__ __launch_bounds__(256) void k_sort_scatter(uint32_t n, PState src, PS
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t d = offsets[keys[s]] + rank[s];
+  dst.rec(d) = src.rec(s);
 #pragma unroll
-  for (int k = 0; k < NPLANES; k++) dst.at(k, d) = src.at(k, s);
-  dst.id[d] = src.id[s];
+  for (int k = VX; k < NPLANES; k++) dst.at(k, d) = src.at(k, s);
   dst.density[d] = src.density[s];
 }
 

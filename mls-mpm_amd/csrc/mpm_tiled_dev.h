@@ -145,7 +145,8 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
   __syncthreads();
   for (uint32_t idx = t; idx < it.z; idx += nt) {
     uint32_t s = it.y + idx;
-    float px = S.at(PX, s), py = S.at(PY, s), pz = S.at(PZ, s);
+    const float4 q = S.rec(s);
+    float px = q.x, py = q.y, pz = q.z;
     PW pw;
     particle_weights(px, py, pz, pw);
     if (ko.counts) {
@@ -634,12 +635,13 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
   auto load_in = [&](uint32_t i, In &in) {
     const uint32_t j = i < it.z ? i : it.z - 1;
     in.s = EXACT ? lsrc[j] : it.y + j;
-    in.px = S.at(PX, in.s);
-    in.py = S.at(PY, in.s);
-    in.pz = S.at(PZ, in.s);
+    const float4 q = S.rec(in.s);  // position and id: one load
+    in.px = q.x;
+    in.py = q.y;
+    in.pz = q.z;
+    in.id = __float_as_uint(q.w);
     if (EXACT) {
       in.m = P.unit_mass ? 1.0f : S.at(PM, in.s);  // unit mass: the plane is never read
-      in.id = S.idr(in.s);
       in.dens = S.dens(in.s);
     }
   };
@@ -663,17 +665,16 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
         return make_float3(v.x, v.y, v.z);
       }, o);
     }
-    store_g2p(D, d, o);
+    store_g2p(D, d, o, cur.id);
     if (EXACT) {
       if (!P.unit_mass) D.at(PM, d) = cur.m;
-      D.idr(d) = cur.id;
       D.dens(d) = cur.dens;
     }
     if (GEN) {
       const float dens = EXACT ? cur.dens : S.dens(cur.s);
       const float m = P.unit_mass ? 1.0f : (EXACT ? cur.m : S.at(PM, cur.s));
       if (dens > 0.0f && dens <= P.density_bubble_min) {
-        const uint32_t id = EXACT ? cur.id : S.idr(cur.s);
+        const uint32_t id = cur.id;
         GenEval e;
         if (gen_decide(P, g, dens, o.vel.x, o.vel.y, o.vel.z, o.pos.x, o.pos.y, o.pos.z, m, id,
                        gen_seed, e))
