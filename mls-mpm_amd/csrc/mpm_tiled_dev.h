@@ -75,6 +75,20 @@ __device__ __forceinline__ int window_cell(const Params &P, int ox, int oy, int 
   return (z * (int)P.gy + y) * (int)P.gx + x;
 }
 
+// window_cell without branches: gi is the cell's global index inside the
+// grid, else 0 (a valid address); returns whether the cell is inside.
+template <int E>
+__device__ __forceinline__ bool window_cell_sel(const Params &P, int ox, int oy, int oz, int c,
+                                                int &gi) {
+  const int wz = c / (E * E), rem = c - wz * (E * E);
+  const int wy = rem / E, wx = rem - wy * E;
+  const int x = ox + wx, y = oy + wy, z = oz + wz;
+  const bool in = ((unsigned)x < P.gx) & ((unsigned)y < P.gy) & ((unsigned)z < P.gz);
+  // a mask, not a select: a select here is compiled into a branch
+  gi = ((z * (int)P.gy + y) * (int)P.gx + x) & -(int)in;
+  return in;
+}
+
 __device__ __forceinline__ void flag_cell_brick(const BrickWork &bw, int x, int y, int z) {
   bw.ofl[brick_index(bw, x >> 3, y >> 3, z >> 3)] = 1;
 }
@@ -110,6 +124,28 @@ __device__ __forceinline__ WinPos win_pos(const PW &pw, int ox, int oy, int oz) 
   return w;
 }
 
+// The thread index as an opaque value.  Unrolled staging loops derive their
+// window offsets from it; in the batched kernels, which run the item bodies
+// in a loop over work units, those offsets would otherwise be hoisted out of
+// the loop and held in registers across every item.
+__device__ __forceinline__ int opaque_lane(int t) {
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+// Work item blk of a launch and whether it exists.  The item buffers hold an
+// entry for every block of a launch (max_items), so the item is loaded
+// together with the count rather than after it.
+__device__ __forceinline__ bool load_item(const uint4 *items, const uint32_t *nitems,
+                                          uint32_t blk, uint4 &it) {
+  const uint32_t n = *nitems;
+  it = items[blk];
+  // Both operands evaluated: the compiler keeps the item load ahead of the
+  // branch instead of sinking it behind the count's round trip.  A count is
+  // never all ones.
+  return (blk < n) & (it.z != 0xffffffffu);
+}
+
 // ---------------------------------------------------------------------------
 // P2G mass over brick work items (WGSL_P2G_MASS, shaders.js:242-274).  With
 // ko.counts set, also assigns every particle its brick and rank for the
@@ -125,8 +161,8 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
   // tags here until the bases are known; a larger one in ko.rank.
   constexpr uint32_t kStageRanks = 8u * kTB;
   __shared__ uint32_t lrk[kStageRanks];
-  if (blk >= *bw.nitems) return;
-  const uint4 it = bw.items[blk];
+  uint4 it;
+  if (!load_item(bw.items, bw.nitems, blk, it)) return;
   int ibx, iby, ibz;
   brick_coords(bw, it.w, ibx, iby, ibz);
   const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
@@ -251,18 +287,40 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
   constexpr int E = 10 + 2 * M, V = E * E * E;
   __shared__ float lm[V];
   __shared__ int32_t lv[3 * V];
-  if (blk >= *bw.nitems) return;
-  const uint4 it = bw.items[blk];
+  uint4 it;
+  if (!load_item(bw.items, bw.nitems, blk, it)) return;
   int ibx, iby, ibz;
   brick_coords(bw, it.w, ibx, iby, ibz);
   const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
   const int t = threadIdx.x, nt = blockDim.x;
-  for (int c = t; c < V; c += nt) {
-    int gi = window_cell<E>(P, ox, oy, oz, c);
-    lm[c] = gi >= 0 ? decode(g.m[gi]) : 0.0f;
-    lv[3 * c] = 0;
-    lv[3 * c + 1] = 0;
-    lv[3 * c + 2] = 0;
+  // The decoded mass window: every load of a thread issued before the first
+  // decode (blocks of kTB threads), so the window costs one memory latency.
+  // The loads are unconditional (cells outside the grid read cell 0 and are
+  // then zeroed): a load under a branch is waited for at the branch's end.
+  constexpr int kPer = (V + kTB - 1) / kTB;
+  int32_t mw[kPer];
+  bool in[kPer];
+  const int tw = opaque_lane(t);
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int c = tw + u * kTB;
+    int gi;
+    in[u] = window_cell_sel<E>(P, ox, oy, oz, c < V ? c : V - 1, gi) && c < V;
+    mw[u] = g.m[gi];
+  }
+  // No decode is scheduled among the loads (that would wait for each one).
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int u = 0; u < kPer; u++) mw[u] = in[u] ? mw[u] : 0;
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int c = tw + u * kTB;
+    if (c < V) {
+      lm[c] = decode(mw[u]);
+      lv[3 * c] = 0;
+      lv[3 * c + 1] = 0;
+      lv[3 * c + 2] = 0;
+    }
   }
   __syncthreads();
   const uint32_t gx = P.gx, slice = P.gx * P.gy;
@@ -584,38 +642,44 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
   __shared__ uint32_t lsrc[EXACT ? kTB * kMaxPpt : 1];
   // A re-sort writes the next work-item set's order; it also empties the
   // resolve list for the next step (no resolve runs after G2P).
+  uint4 it;
+  const bool live =
+      load_item(EXACT ? bw.items_w : bw.items, EXACT ? bw.nitems_w : bw.nitems, blk, it);
   if (EXACT && blk == 0 && threadIdx.x == 0) *bw.nlist = 0u;
-  if (blk >= *(EXACT ? bw.nitems_w : bw.nitems)) return;
-  const uint4 it = (EXACT ? bw.items_w : bw.items)[blk];
+  if (!live) return;
   int ibx, iby, ibz;
   brick_coords(bw, it.w, ibx, iby, ibz);
   const int ox = ibx * 8 - 1 - M, oy = iby * 8 - 1 - M, oz = ibz * 8 - 1 - M;
   const int t = threadIdx.x, nt = blockDim.x;
-  // Four loads in flight per thread, then their stores.
-  for (int c0 = t; c0 < V; c0 += 4 * nt) {
-    float4 v4[4];
+  // The window and the first kPerS rows of source slots are all loaded
+  // before the first is stored (blocks of kTB threads): the staging costs
+  // one memory latency, not one per round.  Items of more than kPerS rows
+  // (16 particles per thread) stage the rest afterwards.
+  constexpr int kPerV = (V + kTB - 1) / kTB, kPerS = EXACT ? 8 : 1;
+  float4 v4[kPerV];
+  uint32_t s4[kPerS];
+  const int tw = opaque_lane(t);
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int c = c0 + u * nt;
-      const int gi = c < V ? window_cell<E>(P, ox, oy, oz, c) : -1;
-      v4[u] = gi >= 0 ? g.v[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (c0 + u * nt < V) lv[c0 + u * nt] = v4[u];
+  for (int u = 0; u < kPerV; u++) {
+    const int c = tw + u * kTB;
+    const int gi = c < V ? window_cell<E>(P, ox, oy, oz, c) : -1;
+    v4[u] = gi >= 0 ? g.v[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
   if (EXACT)
-    for (uint32_t i0 = t; i0 < it.z; i0 += 4u * nt) {
-      uint32_t s4[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const uint32_t i = i0 + u * nt;
-        s4[u] = i < it.z ? src_of[it.y + i] : 0u;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (i0 + u * nt < it.z) lsrc[i0 + u * nt] = s4[u];
+    for (int u = 0; u < kPerS; u++) {
+      const uint32_t i = (uint32_t)tw + u * kTB;
+      s4[u] = i < it.z ? src_of[it.y + i] : 0u;
     }
+#pragma unroll
+  for (int u = 0; u < kPerV; u++)
+    if (tw + u * kTB < V) lv[tw + u * kTB] = v4[u];
+  if (EXACT) {
+#pragma unroll
+    for (int u = 0; u < kPerS; u++)
+      if ((uint32_t)tw + u * kTB < it.z) lsrc[tw + u * kTB] = s4[u];
+    for (uint32_t i = t + kPerS * kTB; i < it.z; i += kTB) lsrc[i] = src_of[it.y + i];
+  }
   __syncthreads();
   uint32_t idx = t;
   if (idx >= it.z) return;
