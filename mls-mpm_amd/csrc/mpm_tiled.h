@@ -19,7 +19,8 @@ struct TileCfg {
 // set (the particle order the P2G passes read) and the next set, written by
 // a re-sort and read by the re-sorting G2P.
 struct BrickWork {
-  const uint4 *items;            // current: (brick rank, first particle, count, brick index)
+  const uint4 *items;            // current, in dispatch order: (ghost core, first particle,
+                                 // count, brick index)
   uint4 *items_w;                // next
   uint32_t *nitems, *nitems_w;   // device counts of items (current, next)
   uint32_t *first, *count;       // current, per brick index: first item and number of items

@@ -258,7 +258,7 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
     }
   }
   // core to the ghost buffer, non-zero margin cells to the overflow grid
-  int32_t *gh = bw.gm + (size_t)blk * kCoreV;
+  int32_t *gh = bw.gm + (size_t)it.x * kCoreV;  // the item's ghost core
   for (int c = t; c < kCoreV; c += nt) {
     int cx, cy, cz;
     ghost_cell(c, cx, cy, cz);
@@ -365,7 +365,7 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
     idx = nidx;
   }
   __syncthreads();
-  const size_t gb = (size_t)blk * kCoreV;
+  const size_t gb = (size_t)it.x * kCoreV;  // the item's ghost core
   for (int c = t; c < kCoreV; c += nt) {
     int cx, cy, cz;
     ghost_cell(c, cx, cy, cz);
@@ -760,28 +760,37 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
 // ---------------------------------------------------------------------------
 // Sort plumbing.
 
-// Work items from the brick offsets: each brick's particle range cut into
-// pieces of at most `ch` particles.  Item order is irrelevant to the result.
-// Work items of a brick (index b, rank r): its range cut into pieces of at
-// most ch = 1 << chs particles.
-__device__ __forceinline__ void write_items(const BrickWork &bw, uint32_t r, uint32_t b,
-                                            uint32_t start, uint32_t cnt, uint32_t chs,
-                                            uint32_t item0) {
+// Work items of a brick (index b): its range cut into pieces of at most
+// ch = 1 << chs particles, numbered item0.. in brick-rank order.  That number
+// (the item's ghost core) goes into the item's first field; the item itself is
+// stored at its dispatch position: items of more rows of kTB particles come
+// first (cur[k] is the next free position among items of R - k rows, R the
+// rows of a full item), so the last blocks of a launch are the shortest.  The
+// order among items of one size is irrelevant to every result.
+__device__ __forceinline__ void write_items(const BrickWork &bw, uint32_t b, uint32_t start,
+                                            uint32_t cnt, uint32_t chs, uint32_t item0,
+                                            uint32_t *cur, uint32_t R) {
   const uint32_t ch = 1u << chs;
-  const uint32_t pieces = (cnt + ch - 1) >> chs;
+  const uint32_t full = cnt >> chs, rest = cnt & (ch - 1u);
   bw.first_w[b] = cnt ? item0 : 0u;
-  bw.count_w[b] = pieces;
+  bw.count_w[b] = full + (rest ? 1u : 0u);
   bw.occ_w[b] = cnt ? 1 : 0;
-  for (uint32_t p = 0; p < pieces; p++) {
-    const uint32_t c = cnt - p * ch;
-    bw.items_w[item0 + p] = make_uint4(r, start + p * ch, c < ch ? c : ch, b);
+  if (full) {
+    const uint32_t pos = atomicAdd(&cur[0], full);
+    for (uint32_t p = 0; p < full; p++)
+      bw.items_w[pos + p] = make_uint4(item0 + p, start + p * ch, ch, b);
+  }
+  if (rest) {
+    const uint32_t pos = atomicAdd(&cur[R - (rest + kTB - 1u) / kTB], 1u);
+    bw.items_w[pos] = make_uint4(item0 + full, start + full * ch, rest, b);
   }
 }
 
 // One block: exclusive scans of the per-brick particle counts (the new
 // brick offsets) and of the per-brick work-item counts, the work items of
-// the new order (numbered in brick-rank order) and their number; the counts
-// are cleared for the next step.  Writes the next work-item set.  A round
+// the new order (numbered in brick-rank order, stored largest first: a first
+// pass over the counts sizes each class) and their number; the counts are
+// cleared for the next step.  Writes the next work-item set.  A round
 // covers K * blockDim bricks, row k of it bricks base + k * blockDim + t, so
 // every load and store of a row is coalesced; the rows are scanned in turn.
 // The block size is a multiple of 64, at most 1024.
@@ -789,9 +798,29 @@ template <int K>
 __device__ __forceinline__ void scan_items_block(const BrickWork &bw, uint32_t chs,
                                                  uint32_t *counts, uint32_t *offsets) {
   __shared__ uint32_t wp[16], wi[16];
-  const uint32_t ch = 1u << chs;
+  // items per size class (class k: R - k rows), then the next free dispatch
+  // position of each class
+  __shared__ uint32_t cur[kMaxPpt];
+  const uint32_t ch = 1u << chs, R = ch / kTB;
   const uint32_t t = threadIdx.x, nt = blockDim.x, lane = t & 63u, wave = t >> 6;
   const uint32_t nw = nt >> 6, nb = bw.nbricks;
+  if (t < (uint32_t)kMaxPpt) cur[t] = 0u;
+  __syncthreads();
+  for (uint32_t r = t; r < nb; r += nt) {
+    const uint32_t c = counts[r], full = c >> chs, rest = c & (ch - 1u);
+    if (full) atomicAdd(&cur[0], full);
+    if (rest) atomicAdd(&cur[R - (rest + kTB - 1u) / kTB], 1u);
+  }
+  __syncthreads();
+  if (t == 0) {
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < R; k++) {
+      const uint32_t v = cur[k];
+      cur[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
   uint32_t carry_p = 0, carry_i = 0;  // running totals, the same in every thread
   for (uint32_t base = 0; base < nb; base += nt * K) {
     uint32_t c[K], bk[K];
@@ -834,7 +863,7 @@ __device__ __forceinline__ void scan_items_block(const BrickWork &bw, uint32_t c
       if (r < nb) {
         offsets[r] = ep;
         counts[r] = 0u;
-        write_items(bw, r, bk[k], ep, c[k], chs, ei);
+        write_items(bw, bk[k], ep, c[k], chs, ei, cur, R);
       }
       carry_p += tp;
       carry_i += ti;
