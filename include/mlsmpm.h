@@ -285,8 +285,9 @@ int mpm_get_stream(mpm_scene *s, void **hip_stream);
  * mpm_enable_timing(s, 1) clears the record and starts it; mpm_stage_times
  * gives the mean per step, in milliseconds, of
  * [p2g_mass, p2g_mom, update, g2p, diffuse, sort, whole step, relayout] over
- * the steps recorded since ("g2p" brackets the G2P kernel alone, "relayout"
- * the offsets/work-item/permutation kernels of a re-sort); mpm_step_times
+ * the steps recorded since ("g2p" brackets the G2P kernel alone; a re-sort's
+ * offsets, work items and permutation run inside the p2g_mass and p2g_mom
+ * stages, so "relayout", the wait for them, is kept at zero); mpm_step_times
  * gives each recorded step's total (up to cap values; *count = number
  * recorded).  Both synchronise. */
 int mpm_enable_timing(mpm_scene *s, int enable);

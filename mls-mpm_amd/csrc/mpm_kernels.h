@@ -41,6 +41,9 @@ struct SortBufs {
   uint32_t *scratch;    // scan scratch
   uint32_t *brick_rank; // 8x8x8 grid brick -> position in Morton order
   uint32_t *src_of;     // per new slot: old slot (G2P re-layout)
+  uint32_t *gsum;       // re-sort scan: per group of 64 brick ranks, its bases
+  uint32_t *gacc;       // re-sort scan: running totals
+  uint2 *spans;         // re-sort scan: per brick rank, [first, end) slots
   uint32_t nkeys;       // number of bricks
 };
 

@@ -28,10 +28,6 @@ struct mpm_scene {
   uint32_t *nitems_buf[2] = {nullptr, nullptr};
   uint32_t *first_buf[2] = {nullptr, nullptr}, *count_buf[2] = {nullptr, nullptr};
   uint8_t *occ_buf[2] = {nullptr, nullptr};
-  // the re-sort preparation runs on `side`, forked after P2G mass and joined
-  // before G2P
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   uint32_t *brick_of_rank = nullptr;
   mpm::SortBufs sort{};
   mpm::Walls walls{};

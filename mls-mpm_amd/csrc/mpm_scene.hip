@@ -85,7 +85,8 @@ size_t scene_footprint(const mpm_config &c) {
   size_t b = 0;
   b += 2 * cap * (kStateFloats * 4 + 4);  // particle records and planes, density (two orders)
   b += G * (8 * 4 + 16);                 // i32 grids, overflow grids, float4 velocity/mass
-  b += cap * 12 + nb * 16;               // keys, ranks, src_of; counts, offsets, first, count
+  b += cap * 12 + nb * 24;               // keys, ranks, src_of; counts, offsets, first, count, spans
+  b += (nb / 64 + 1) * 18 * 4 + 2304;    // re-sort scan: group bases, running totals
   b += nit * (16 + 4 * 4000);            // work items and ghost cores
   b += nb * 8 + 65536;                   // flags, lists, scan scratch
   b += cap * 72;                         // read-back staging
@@ -257,9 +258,6 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     return failf(MPM_ERR_INVALID, "device %d out of range (%d devices)", c.device, ndev);
   HIPCHK(hipSetDevice(c.device));
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
   const size_t cap = c.capacity ? c.capacity : 1;
   s->G = c.grid_x * c.grid_y * c.grid_z;
   const size_t G = s->G;
@@ -315,6 +313,10 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     DALLOC(s->sort.offsets, (size_t)s->sort.nkeys * 4);
     DALLOC(s->sort.scratch, (size_t)scan_scratch_words(s->sort.nkeys) * 4);
     DALLOC(s->sort.src_of, cap * 4);
+    const size_t ngroups = (s->sort.nkeys + 63) / 64;
+    DALLOC(s->sort.gsum, ngroups * kScanStride * 4);
+    DALLOC(s->sort.gacc, kScanStride * kAccStride * 4);
+    DALLOC(s->sort.spans, (size_t)s->sort.nkeys * 8);
     HIPCHK(hipMemsetAsync(s->sort.counts, 0, (size_t)s->sort.nkeys * 4, s->stream));
     std::vector<uint32_t> rank = morton_tile_rank(bw.nbx, bw.nby, bw.nbz);
     std::vector<uint32_t> inv(rank.size());
@@ -379,11 +381,6 @@ void scene_free(mpm_scene *s) {
   s->stage = nullptr;
   for (int i = 0; i < mpm_scene::kEvPerStep * mpm_scene::kEvSteps; i++)
     if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
-  if (s->side) (void)hipStreamSynchronize(s->side);
-  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
-  if (s->ev_join) (void)hipEventDestroy(s->ev_join);
-  if (s->side) (void)hipStreamDestroy(s->side);
-  s->side = nullptr;
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
 }
@@ -485,7 +482,8 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
 }
 
 // Add one step's 8 events (start, sort, P2G mass, P2G momentum, update,
-// relayout, G2P, diffuse) to the stage sums.
+// relayout (an empty interval: the re-sort's preparation runs inside the P2G
+// stages), G2P, diffuse) to the stage sums.
 static int fold_step_events(mpm_scene *s, hipEvent_t *e) {
   const int E = mpm_scene::kEvPerStep;
   float ms[E - 1];
@@ -528,7 +526,7 @@ static int fold_timing(mpm_scene *s) {
 }
 
 // Enqueue one step.  e (optional) receives 8 events: start, after the sort,
-// P2G mass, P2G momentum, grid update, relayout preparation, G2P, diffuse.
+// P2G mass, P2G momentum, grid update, (empty) relayout, G2P, diffuse.
 // When captured (graph_events), the events become event-record nodes.
 static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) {
   const bool keep = (s->cfg.flags & MPM_FLAG_KEEP_GRID) != 0;
@@ -575,21 +573,20 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   // zero only with KEEP_GRID off and outside the bricks it resolved
   if ((keep || lds) && !tiled) launch_clear_grid(s->g, s->G, st);
   KeyOut ko{};
-  if (relayout) ko = key_out(s->sort, bw);
+  if (relayout) {
+    ko = key_out(s->sort, bw);
+    ko.acc = s->sort.gacc;
+  }
   if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, bw, ko, st);
   else launch_p2g_mass_global(P, S, s->g, st);
-  if (relayout) {
-    // the re-sort preparation (offsets, next work items, inverse
-    // permutation) needs only the mass pass: it overlaps the resolves and
-    // P2G momentum on the side stream
-    HIPCHK(hipEventRecord(s->ev_fork, st));
-    HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
-    launch_relayout_prep(P, s->sort, bw, s->tc, s->side);
-    HIPCHK(hipEventRecord(s->ev_join, s->side));
-  }
-  if (tiled) launch_resolve_mass(P, s->g, bw, st);
+  // A re-sort's scan (offsets, next work items) runs inside the brick-list
+  // and mass-resolve kernels and its inverse permutation inside P2G
+  // momentum: one stream, no extra launches.
+  if (tiled) launch_resolve_mass(P, s->g, bw, st, relayout ? scan_args(s->sort, s->tc) : ScanArgs{});
   if ((rc = mark(2))) return rc;
-  if (tiled) launch_p2g_mom_tiled(P, S, s->g, s->tc, bw, st);
+  if (tiled)
+    launch_p2g_mom_tiled(P, S, s->g, s->tc, bw, st,
+                         relayout ? src_out(s->sort, s->tc) : SrcOut{});
   else launch_p2g_mom_global(P, S, s->g, st);
   if ((rc = mark(3))) return rc;
   if (tiled) launch_resolve_update(P, s->g, s->tc, bw, s->walls, keep, st);
@@ -597,7 +594,6 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   if (s->hand_on)
     launch_apply_hand(P, s->g, lds ? s->bw.act : nullptr, s->bw.nbx, s->bw.nby, s->hand_dev, st);
   if ((rc = mark(4))) return rc;
-  if (relayout) HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
   if ((rc = mark(5))) return rc;
   // with a diffuse pool, the tiled G2P also decides the spawners
   const GenOut go = (tiled && P.dmax) ? GenOut{s->d.seed, s->d.genbits, s->d.genval} : GenOut{};

@@ -37,10 +37,37 @@ struct BrickWork {
   uint32_t nbx, nby, nbz, nbricks;
 };
 
+// The re-sort's scan of the per-brick counts, spread over the step's own
+// kernels: brick ranks in groups of 64 (one wavefront).  The brick-list
+// kernel sums each group (particles, work items, items per size class) and
+// reserves the group's ranges with one atomic per quantity on running
+// totals (zeroed by the mass pass); the mass resolve kernel then writes the
+// group's brick spans and work items.  Groups take their ranges in arrival
+// order: each brick's particles stay contiguous, the bricks of a group
+// consecutive.  counts == nullptr: no re-sort this step.
+constexpr uint32_t kScanStride = 18;  // quantities: 2 + the largest particles-per-thread
+constexpr uint32_t kAccStride = 32;   // words between running totals (one cache line each)
+struct ScanArgs {
+  uint32_t *counts;  // per brick rank, cleared by the second phase
+  uint2 *spans;      // per brick rank: [first, end) slots of the brick in the new order
+  uint32_t *gsum;    // per group, kScanStride words: the group's reserved bases
+  uint32_t *acc;     // kScanStride running totals, kAccStride words apart
+  uint32_t chs;      // log2 of the work-item size
+};
+// The inverse permutation of the re-sort, written by the momentum pass (it
+// reads every particle of the current order after the scan has run).
+struct SrcOut {
+  const uint32_t *keys, *rank;
+  const uint2 *spans;
+  uint32_t *src_of;
+  uint32_t chs;
+};
+
 // Where the P2G mass pass records each particle's new brick (Morton rank) and
 // its rank among that brick's particles (counts == nullptr: do not record).
 struct KeyOut {
   uint32_t *counts, *keys, *rank;
+  uint32_t *acc;  // set: the pass zeroes the re-sort scan's running totals (ScanArgs)
 };
 
 bool tiled_ppt_supported(uint32_t ppt);
@@ -52,11 +79,16 @@ KeyOut key_out(const SortBufs &sb, const BrickWork &bw);
 // it, and a step without a re-sort ends with launch_clear_list.
 void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
                            const BrickWork &bw, const KeyOut &ko, hipStream_t st);
-// The resolve list (bricks to resolve) and the mass resolve.
-void launch_resolve_mass(const Params &P, const GState &g, const BrickWork &bw, hipStream_t st);
+// The resolve list (bricks to resolve) and the mass resolve; with sa.counts
+// set, also the re-sort's scan (offsets, next work items and their count).
+void launch_resolve_mass(const Params &P, const GState &g, const BrickWork &bw, hipStream_t st,
+                         const ScanArgs &sa = ScanArgs{});
+ScanArgs scan_args(const SortBufs &sb, const TileCfg &tc);
+SrcOut src_out(const SortBufs &sb, const TileCfg &tc);
 void launch_clear_list(const BrickWork &bw, hipStream_t st);
+// so.src_of set: the pass also writes the re-sort's inverse permutation.
 void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
-                          const BrickWork &bw, hipStream_t st);
+                          const BrickWork &bw, hipStream_t st, const SrcOut &so = SrcOut{});
 void launch_resolve_update(const Params &P, const GState &g, const TileCfg &tc,
                            const BrickWork &bw, const Walls &W, bool keep, hipStream_t st);
 // go.bits set: the pass also decides the diffuse spawners (launch_diffuse
@@ -66,7 +98,5 @@ void launch_g2p_tiled(const Params &P, const PState &S, const PState &D, const G
                       const uint32_t *src_of, hipStream_t st, const GenOut &go = GenOut{});
 void launch_sort(const Params &P, const PState &src, const PState &dst, const SortBufs &sb,
                  const BrickWork &bw, const TileCfg &tc, hipStream_t st);
-void launch_relayout_prep(const Params &P, const SortBufs &sb, const BrickWork &bw,
-                          const TileCfg &tc, hipStream_t st);
 
 }  // namespace mpm

@@ -42,23 +42,27 @@ __global__ __launch_bounds__(256) void k_p2g_mass_brick(Params P, PState S, Bric
   p2g_mass_item<M, UNIT>(P, S, bw, ko, blockIdx.x);
 }
 
-template <int M, bool UNIT>
+template <int M, bool UNIT, bool SRC>
 __global__ __launch_bounds__(256) void k_p2g_mom_brick(Params P, PState S, GState g,
-                                                       BrickWork bw) {
-  p2g_mom_item<M, UNIT>(P, S, g, bw, blockIdx.x);
+                                                       BrickWork bw, SrcOut so) {
+  p2g_mom_item<M, UNIT, SRC>(P, S, g, bw, blockIdx.x, so);
 }
 
 // One wavefront per 64 bricks: its needed bricks go to the list together,
-// in brick order, with one atomic (the list follows the grid).
-__global__ __launch_bounds__(64) void k_brick_list(BrickWork bw) {
+// in brick order, with one atomic (the list follows the grid).  With
+// sa.counts set, the same wavefront also sums the group of 64 brick ranks
+// of the re-sort's scan (first phase).
+__global__ __launch_bounds__(64) void k_brick_list(BrickWork bw, ScanArgs sa) {
   const uint32_t lane = threadIdx.x, b = blockIdx.x * 64u + lane;
   const bool need = brick_needed(bw, b);
   const uint64_t m = __ballot(need);
-  if (!m) return;
-  uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(bw.nlist, (uint32_t)__popcll(m));
-  base = __shfl(base, 0, 64);
-  if (need) bw.list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = b;
+  if (m) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(bw.nlist, (uint32_t)__popcll(m));
+    base = __shfl(base, 0, 64);
+    if (need) bw.list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = b;
+  }
+  if (sa.counts) scan_group_sums(bw, sa, blockIdx.x);
 }
 
 
@@ -75,14 +79,13 @@ __global__ __launch_bounds__(1024) void k_scan_items(BrickWork bw, uint32_t chs,
   scan_items_block<8>(bw, chs, counts, offsets);
 }
 
-__global__ __launch_bounds__(256) void k_build_src(uint32_t n, uint32_t nbricks, uint32_t ch,
-                                                   const uint32_t *keys, const uint32_t *rank,
-                                                   const uint32_t *offsets, uint32_t *src_of) {
-  build_src_one(n, nbricks, ch, keys, rank, offsets, src_of, blockIdx.x * blockDim.x + threadIdx.x);
-}
-
+// With sa.counts set, the first wavefront of each block first writes groups of
+// the re-sort's scan (second phase), one group per block and round.
 __global__ __launch_bounds__(kResolveThreads) void k_resolve_mass(Params P, GState g,
-                                                                  BrickWork bw) {
+                                                                  BrickWork bw, ScanArgs sa) {
+  if (sa.counts && threadIdx.x < 64u)
+    for (uint32_t q = blockIdx.x; q < (bw.nbricks + 63u) >> 6; q += gridDim.x)
+      scan_group_write(bw, sa, q);
   const uint32_t nl = *bw.nlist;
   for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x)
     resolve_mass_entry<kResolveCells>(P, g, bw, bw.list[i]);
@@ -223,24 +226,51 @@ void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, co
   (void)g;
 }
 
-void launch_resolve_mass(const Params &P, const GState &g, const BrickWork &bw, hipStream_t st) {
+void launch_resolve_mass(const Params &P, const GState &g, const BrickWork &bw, hipStream_t st,
+                         const ScanArgs &sa) {
   if (!P.n) return;
   // 64-thread blocks: one brick per thread spread over more CUs
-  k_brick_list<<<nblk(bw.nbricks, 64), 64, 0, st>>>(bw);
-  k_resolve_mass<<<resolve_grid(bw), kResolveThreads, 0, st>>>(P, g, bw);
+  k_brick_list<<<nblk(bw.nbricks, 64), 64, 0, st>>>(bw, sa);
+  k_resolve_mass<<<resolve_grid(bw), kResolveThreads, 0, st>>>(P, g, bw, sa);
+}
+
+ScanArgs scan_args(const SortBufs &sb, const TileCfg &tc) {
+  ScanArgs sa;
+  sa.counts = sb.counts;
+  sa.spans = sb.spans;
+  sa.gsum = sb.gsum;
+  sa.acc = sb.gacc;
+  sa.chs = log2_pow2(kTB * tc.ppt);
+  return sa;
+}
+
+SrcOut src_out(const SortBufs &sb, const TileCfg &tc) {
+  SrcOut so;
+  so.keys = sb.keys;
+  so.rank = sb.rank;
+  so.spans = sb.spans;
+  so.src_of = sb.src_of;
+  so.chs = log2_pow2(kTB * tc.ppt);
+  return so;
 }
 
 void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,
-                          const BrickWork &bw, hipStream_t st) {
+                          const BrickWork &bw, hipStream_t st, const SrcOut &so) {
   if (!P.n) return;
   uint32_t grid = max_items(P.n, bw.nbricks, tc.ppt), tb = tc.threads;
+#define MOM_LAUNCH(M_, U_)                                                                  \
+  do {                                                                                      \
+    if (so.src_of) k_p2g_mom_brick<M_, U_, true><<<grid, tb, 0, st>>>(P, S, g, bw, so);     \
+    else k_p2g_mom_brick<M_, U_, false><<<grid, tb, 0, st>>>(P, S, g, bw, so);              \
+  } while (0)
   if (tc.margin == 1) {
-    if (P.unit_mass) k_p2g_mom_brick<1, true><<<grid, tb, 0, st>>>(P, S, g, bw);
-    else k_p2g_mom_brick<1, false><<<grid, tb, 0, st>>>(P, S, g, bw);
+    if (P.unit_mass) MOM_LAUNCH(1, true);
+    else MOM_LAUNCH(1, false);
   } else {
-    if (P.unit_mass) k_p2g_mom_brick<2, true><<<grid, tb, 0, st>>>(P, S, g, bw);
-    else k_p2g_mom_brick<2, false><<<grid, tb, 0, st>>>(P, S, g, bw);
+    if (P.unit_mass) MOM_LAUNCH(2, true);
+    else MOM_LAUNCH(2, false);
   }
+#undef MOM_LAUNCH
 }
 
 void launch_resolve_update(const Params &P, const GState &g, const TileCfg &tc,
@@ -273,14 +303,8 @@ KeyOut key_out(const SortBufs &sb, const BrickWork &bw) {
   ko.counts = sb.counts;
   ko.keys = sb.keys;
   ko.rank = sb.rank;
+  ko.acc = nullptr;
   return ko;
-}
-
-static void offsets_items_src(const Params &P, const SortBufs &sb, const BrickWork &bw,
-                              const TileCfg &tc, hipStream_t st) {
-  k_scan_items<<<1, 1024, 0, st>>>(bw, log2_pow2(kTB * tc.ppt), sb.counts, sb.offsets);
-  k_build_src<<<nblk(P.n, 256), 256, 0, st>>>(P.n, bw.nbricks, kTB * tc.ppt, sb.keys, sb.rank,
-                                              sb.offsets, sb.src_of);
 }
 
 // Full sort for the first step after a reset: brick keys, offsets, work items
@@ -293,17 +317,8 @@ void launch_sort(const Params &P, const PState &src, const PState &dst, const So
   k_sort_scatter<<<nblk(P.n, 256), 256, 0, st>>>(P.n, src, dst, sb.keys, sb.rank, sb.offsets);
 }
 
-// After a P2G mass pass that recorded keys: offsets, the next step's work
-// items and the inverse permutation for the re-sorting G2P.  Leaves the
-// counts zeroed.
 void launch_clear_list(const BrickWork &bw, hipStream_t st) {
   k_zero_u32<<<1, 1, 0, st>>>(bw.nlist);
-}
-
-void launch_relayout_prep(const Params &P, const SortBufs &sb, const BrickWork &bw,
-                          const TileCfg &tc, hipStream_t st) {
-  if (!P.n) return;
-  offsets_items_src(P, sb, bw, tc, st);
 }
 
 }  // namespace mpm

@@ -124,6 +124,30 @@ __device__ __forceinline__ WinPos win_pos(const PW &pw, int ox, int oy, int oz) 
   return w;
 }
 
+// floor(r / d) for r < 2^13 and 1 <= d <= 16.  (r + 1/2) / d lies at least
+// 1/32 from every integer, and the float product is within 2^-21 * 2^13 of
+// it, so the truncation is exact (the reciprocal may be off by an ulp).
+__device__ __forceinline__ uint32_t div_small(uint32_t r, uint32_t d) {
+  return (uint32_t)(((float)r + 0.5f) * __builtin_amdgcn_rcpf((float)d));
+}
+
+// Slot of the particle of rank r among the c particles of an item: ranks are
+// stored column-major over rows of 256 slots.  With K = ceil(c/256), the
+// first L = c - 256(K-1) columns hold K ranks, the others K-1.
+__device__ __forceinline__ uint32_t item_slot(uint32_t r, uint32_t c) {
+  const uint32_t K = (c + 255u) >> 8, L = c - 256u * (K - 1u);
+  uint32_t t, k;
+  if (r < L * K) {
+    t = div_small(r, K);
+    k = r - t * K;
+  } else {  // K >= 2 here
+    const uint32_t r2 = r - L * K, q = div_small(r2, K - 1u);
+    t = L + q;
+    k = r2 - q * (K - 1u);
+  }
+  return k * 256u + t;
+}
+
 // The thread index as an opaque value.  Unrolled staging loops derive their
 // window offsets from it; in the batched kernels, which run the item bodies
 // in a loop over work units, those offsets would otherwise be hoisted out of
@@ -161,6 +185,9 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
   // tags here until the bases are known; a larger one in ko.rank.
   constexpr uint32_t kStageRanks = 8u * kTB;
   __shared__ uint32_t lrk[kStageRanks];
+  // the re-sort scan's running totals start every step at zero (the brick
+  // list kernel after this pass adds to them)
+  if (ko.acc && blk == 0 && threadIdx.x < kScanStride) ko.acc[threadIdx.x * kAccStride] = 0u;
   uint4 it;
   if (!load_item(bw.items, bw.nitems, blk, it)) return;
   int ibx, iby, ibz;
@@ -280,10 +307,13 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
 
 // ---------------------------------------------------------------------------
 // P2G momentum over brick work items (WGSL_P2G_MOM, shaders.js:286-370): the
-// decoded mass window and the momentum window live in LDS.
-template <int M, bool UNIT>
+// decoded mass window and the momentum window live in LDS.  SRC: the pass
+// also writes the re-sort's inverse permutation, so.src_of[new slot] = old
+// slot, from each particle's new brick and rank (the scan has run).
+template <int M, bool UNIT, bool SRC = false>
 __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, const GState &g,
-                                             const BrickWork &bw, uint32_t blk) {
+                                             const BrickWork &bw, uint32_t blk,
+                                             const SrcOut &so = SrcOut{}) {
   constexpr int E = 10 + 2 * M, V = E * E * E;
   __shared__ float lm[V];
   __shared__ int32_t lv[3 * V];
@@ -329,11 +359,26 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
   // (in bounds, unconditional).
   uint32_t idx = t;
   MomIn cur;
-  if (idx < it.z) load_mom_in<UNIT>(S, it.y + idx, cur);
+  uint32_t ck = 0, cr = 0;  // SRC: the particle's new brick (Morton rank) and rank in it
+  if (idx < it.z) {
+    load_mom_in<UNIT>(S, it.y + idx, cur);
+    if (SRC) {
+      ck = so.keys[it.y + idx];
+      cr = so.rank[it.y + idx];
+    }
+  }
   while (idx < it.z) {
     const uint32_t s = it.y + idx, nidx = idx + nt;
+    const uint32_t sn = it.y + (nidx < it.z ? nidx : it.z - 1);
     MomIn nxt;
-    load_mom_in<UNIT>(S, it.y + (nidx < it.z ? nidx : it.z - 1), nxt);
+    load_mom_in<UNIT>(S, sn, nxt);
+    uint32_t nk = 0, nr = 0;
+    uint2 span = make_uint2(0u, 0u);  // the new brick's [first, end) slots
+    if (SRC) {
+      nk = so.keys[sn];
+      nr = so.rank[sn];
+      span = so.spans[ck];
+    }
     PW pw;
     particle_weights(cur.px, cur.py, cur.pz, pw);
     WinPos wp = win_pos<M>(pw, ox, oy, oz);
@@ -361,7 +406,13 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
           });
     }
     S.dens(s) = density;
+    if (SRC) {
+      const uint32_t ch = 1u << so.chs, istart = span.x + (cr & ~(ch - 1u));
+      so.src_of[istart + item_slot(cr & (ch - 1u), min(span.y - istart, ch))] = s;
+    }
     cur = nxt;
+    ck = nk;
+    cr = nr;
     idx = nidx;
   }
   __syncthreads();
@@ -758,6 +809,102 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
 }
 
 // ---------------------------------------------------------------------------
+// The re-sort's scan inside the step's kernels (ScanArgs): groups of 64
+// brick ranks, kScanStride words per group: particles, work items, then the
+// items of each size class (class k: R - k rows of kTB particles, R the rows
+// of a full item; dispatch order is class 0 first).
+static_assert(kScanStride == 2u + (uint32_t)kMaxPpt, "scan group stride");
+
+__device__ __forceinline__ uint32_t wave_incl(uint32_t v, uint32_t lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(v, off, 64);
+    if (lane >= (uint32_t)off) v += u;
+  }
+  return v;
+}
+
+// A brick's particles in items: full ones, the rest, and the rest's class
+// (kMaxPpt: no partial item).
+struct BrickPieces {
+  uint32_t c, full, rest, cls;
+};
+__device__ __forceinline__ BrickPieces brick_pieces(uint32_t c, uint32_t chs) {
+  const uint32_t ch = 1u << chs, R = ch / kTB;
+  BrickPieces b;
+  b.c = c;
+  b.full = c >> chs;
+  b.rest = c & (ch - 1u);
+  b.cls = b.rest ? R - (b.rest + kTB - 1u) / kTB : (uint32_t)kMaxPpt;
+  return b;
+}
+
+// Phase 1, one wavefront (the brick-list kernel's block g): the group's sums,
+// and its bases reserved on the running totals (lane q adds quantity q).
+__device__ __forceinline__ void scan_group_sums(const BrickWork &bw, const ScanArgs &sa,
+                                                uint32_t g) {
+  const uint32_t lane = threadIdx.x & 63u, nb = bw.nbricks;
+  const uint32_t R = (1u << sa.chs) / kTB;
+  const uint32_t r = g * 64u + lane;
+  const BrickPieces b = brick_pieces(r < nb ? sa.counts[r] : 0u, sa.chs);
+  uint32_t sum[kScanStride];
+  sum[0] = __shfl(wave_incl(b.c, lane), 63, 64);
+  sum[1] = __shfl(wave_incl(b.full + (b.rest ? 1u : 0u), lane), 63, 64);
+  sum[2] = __shfl(wave_incl(b.full + (b.cls == 0u ? 1u : 0u), lane), 63, 64);
+#pragma unroll
+  for (uint32_t k = 1; k < (uint32_t)kMaxPpt; k++)
+    sum[2 + k] = k < R ? (uint32_t)__popcll(__ballot(b.cls == k)) : 0u;
+  // lane q reserves quantity q: one atomic instruction for the group
+  uint32_t mine = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kScanStride; q++) mine = lane == q ? sum[q] : mine;
+  if (lane < kScanStride)
+    sa.gsum[(size_t)g * kScanStride + lane] = atomicAdd(&sa.acc[lane * kAccStride], mine);
+}
+
+// Phase 2, one wavefront per group (the mass resolve kernel): the group's
+// bases (its reserved ranges; class ranges also offset by the totals of the
+// larger classes), then the brick spans, each brick's first item / item
+// count / occupancy in the next set, and the work items at their dispatch
+// positions (class range of the group plus the rank among the group's items
+// of that class); the counts are cleared, and group 0 writes the item count.
+__device__ __forceinline__ void scan_group_write(const BrickWork &bw, const ScanArgs &sa,
+                                                 uint32_t g) {
+  const uint32_t lane = threadIdx.x & 63u, nb = bw.nbricks;
+  const uint32_t ch = 1u << sa.chs, R = ch / kTB;
+  const uint32_t *gs = sa.gsum + (size_t)g * kScanStride;
+  if (g == 0u && lane == 0) *bw.nitems_w = sa.acc[kAccStride];
+  const uint32_t r = g * 64u + lane;
+  const bool in = r < nb;
+  const BrickPieces b = brick_pieces(in ? sa.counts[r] : 0u, sa.chs);
+  if (in) sa.counts[r] = 0u;
+  const uint32_t pieces = b.full + (b.rest ? 1u : 0u);
+  const uint32_t ep = gs[0] + wave_incl(b.c, lane) - b.c;
+  const uint32_t ei = gs[1] + wave_incl(pieces, lane) - pieces;
+  // class k's range: the larger classes' totals, then the group's base
+  const uint32_t n0 = b.full + (b.cls == 0u ? 1u : 0u);
+  const uint32_t p0 = gs[2] + wave_incl(n0, lane) - n0;
+  uint32_t pp = p0 + b.full;  // the partial item's position (class 0: after the full ones)
+  uint32_t cb = sa.acc[2u * kAccStride];
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t k = 1; k < R; k++) {
+    const uint64_t m = __ballot(b.cls == k);
+    if (b.cls == k) pp = cb + gs[2 + k] + (uint32_t)__popcll(m & below);
+    cb += sa.acc[(2u + k) * kAccStride];
+  }
+  if (in) {
+    sa.spans[r] = make_uint2(ep, ep + b.c);
+    const uint32_t bk = bw.brick_of_rank[r];
+    bw.first_w[bk] = b.c ? ei : 0u;
+    bw.count_w[bk] = pieces;
+    bw.occ_w[bk] = b.c ? 1 : 0;
+    for (uint32_t p = 0; p < b.full; p++)
+      bw.items_w[p0 + p] = make_uint4(ei + p, ep + p * ch, ch, bk);
+    if (b.rest) bw.items_w[pp] = make_uint4(ei + b.full, ep + b.full * ch, b.rest, bk);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Sort plumbing.
 
 // Work items of a brick (index b): its range cut into pieces of at most
@@ -873,24 +1020,6 @@ __device__ __forceinline__ void scan_items_block(const BrickWork &bw, uint32_t c
 }
 
 
-
-// Slot of the particle of rank r among the c particles of an item: ranks are
-// stored column-major over rows of 256 slots.  With K = ceil(c/256), the
-// first L = c - 256(K-1) columns hold K ranks, the others K-1.
-__device__ __forceinline__ uint32_t item_slot(uint32_t r, uint32_t c) {
-  uint32_t K = (c + 255u) / 256u;
-  uint32_t L = c - 256u * (K - 1u);
-  uint32_t t, k;
-  if (r < L * K) {
-    t = r / K;
-    k = r - t * K;
-  } else {
-    uint32_t r2 = r - L * K;
-    t = L + r2 / (K - 1u);
-    k = r2 % (K - 1u);
-  }
-  return k * 256u + t;
-}
 
 // Inverse permutation of the re-sort: src_of[new slot] = old slot.
 __device__ __forceinline__ void build_src_one(uint32_t n, uint32_t nbricks, uint32_t ch,
