@@ -182,9 +182,13 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
   // the 27 neighbour bricks: particle counts, reserved bases, Morton ranks
   __shared__ uint32_t lcnt[27], lbase[27], ldest[27];
   // An item of at most kStageRanks particles keeps its (neighbour, rank)
-  // tags here until the bases are known; a larger one in ko.rank.
+  // tags here until the bases are known, 16 bits each: neighbour << 11 |
+  // rank among the item's particles bound there (< 2048); a particle beyond
+  // the 27 bricks takes its final rank in ko.rank at once and the tag kFar.
+  // A larger item keeps (neighbour << 26 | rank) tags in ko.rank.  The
+  // 16-bit tags keep the block's LDS small enough for 8 blocks per CU.
   constexpr uint32_t kStageRanks = 8u * kTB;
-  __shared__ uint32_t lrk[kStageRanks];
+  __shared__ uint16_t lrk[kStageRanks];
   // the re-sort scan's running totals start every step at zero (the brick
   // list kernel after this pass adds to them)
   if (ko.acc && blk == 0 && threadIdx.x < kScanStride) ko.acc[threadIdx.x * kAccStride] = 0u;
@@ -238,8 +242,12 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
       } else {
         r = atomicAdd(&ko.counts[dest], 1u);
       }
-      if (stage) lrk[idx] = (nb << 26) | r;
-      else ko.rank[s] = (nb << 26) | r;
+      if (stage) {
+        lrk[idx] = (uint16_t)(near ? (nb << 11) | r : (kFar << 11));
+        if (!near) ko.rank[s] = r;
+      } else {
+        ko.rank[s] = (nb << 26) | r;
+      }
     }
     float pm = UNIT ? 1.0f : S.at(PM, s);
     WinPos wp = win_pos<M>(pw, ox, oy, oz);
@@ -279,9 +287,13 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
     __syncthreads();
     for (uint32_t idx = t; idx < it.z; idx += nt) {
       uint32_t s = it.y + idx;
-      uint32_t r = stage ? lrk[idx] : ko.rank[s], nb = r >> 26;
-      if (nb != kFar) ko.rank[s] = lbase[nb] + (r & 0x3ffffffu);
-      else ko.rank[s] = r & 0x3ffffffu;
+      if (stage) {
+        const uint32_t v = lrk[idx], nb = v >> 11;
+        if (nb != kFar) ko.rank[s] = lbase[nb] + (v & 0x7ffu);
+      } else {
+        const uint32_t r = ko.rank[s], nb = r >> 26;
+        ko.rank[s] = (nb != kFar ? lbase[nb] : 0u) + (r & 0x3ffffffu);
+      }
     }
   }
   // core to the ghost buffer, non-zero margin cells to the overflow grid
