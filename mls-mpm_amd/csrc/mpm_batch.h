@@ -54,7 +54,24 @@ struct BatchTables {
   // (capacity tot_bricks), and its length; empty at every step start.
   uint2 *list;
   uint32_t *nlist;
+  // Dispatch order over all scenes of the current work-item set and of the
+  // set the step's re-sort writes: (scene, item) pairs, items with more rows
+  // of 256 particles first (capacity tot_items); with the totals per row
+  // class and in all (kMaxPpt + 1 counters, kAccStride words apart, the
+  // item count last).  ocls: per scene, the bases reserved per class.
+  const uint2 *order_cur;
+  const uint32_t *tot_cur;
+  uint2 *order_next;
+  uint32_t *tot_next;
+  uint32_t *ocls;
 };
+
+// Count of the items of an order (its last counter).
+constexpr uint32_t kOrderTotal = 16u;  // = kMaxPpt
+
+// The dispatch order of a work-item set (the current one, or the one the
+// re-sort writes: next) from the members' item lists.
+void launch_batch_order(const BatchTables &t, bool next, hipStream_t st);
 
 // One batched step (sort period 1: every step re-sorts inside G2P) of every
 // scene of the table, all with drift margin `margin`.

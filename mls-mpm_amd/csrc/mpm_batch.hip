@@ -35,17 +35,22 @@ __device__ __forceinline__ uint32_t find_scene(const uint32_t *__restrict__ pre,
   return lo;
 }
 
+// The item stages walk a dispatch order over all scenes (BatchTables): unit
+// u < the order's count is item order[u] = (scene, item), largest items
+// first, so the launch ends on the shortest blocks of every scene together.
+
 // UNIT: every scene of the table has unit particle mass (the general body
 // gives the same bits for unit mass, multiplying by 1.0f exactly).
 template <int M, bool UNIT>
 __global__ __launch_bounds__(256) void k_batch_mass(const SceneDev *__restrict__ sd,
-                                                    const uint32_t *__restrict__ pre, uint32_t ns,
-                                                    uint32_t total) {
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const uint32_t si = find_scene(pre, ns, u);
+                                                    const uint2 *__restrict__ order,
+                                                    const uint32_t *__restrict__ tot) {
+  const uint32_t nu = tot[kOrderTotal * kAccStride];
+  for (uint32_t u = blockIdx.x; u < nu; u += gridDim.x) {
+    const uint2 e = order[u];
+    const uint32_t si = __builtin_amdgcn_readfirstlane(e.x), j = __builtin_amdgcn_readfirstlane(e.y);
     const SceneDev &d = sd[si];
-    const uint32_t blk = u - pre[si];
-    p2g_mass_item<M, UNIT>(d.P, d.S, d.bw, d.ko, blk);
+    p2g_mass_item<M, UNIT>(d.P, d.S, d.bw, d.ko, j);
     __syncthreads();
   }
 }
@@ -54,28 +59,110 @@ __global__ __launch_bounds__(256) void k_batch_mass(const SceneDev *__restrict__
 // gives the same bits for unit mass, multiplying by 1.0f exactly).
 template <int M, bool UNIT>
 __global__ __launch_bounds__(256) void k_batch_mom(const SceneDev *__restrict__ sd,
-                                                   const uint32_t *__restrict__ pre, uint32_t ns,
-                                                   uint32_t total) {
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const uint32_t si = find_scene(pre, ns, u);
+                                                   const uint2 *__restrict__ order,
+                                                   const uint32_t *__restrict__ tot) {
+  const uint32_t nu = tot[kOrderTotal * kAccStride];
+  for (uint32_t u = blockIdx.x; u < nu; u += gridDim.x) {
+    const uint2 e = order[u];
+    const uint32_t si = __builtin_amdgcn_readfirstlane(e.x), j = __builtin_amdgcn_readfirstlane(e.y);
     const SceneDev &d = sd[si];
-    const uint32_t blk = u - pre[si];
-    p2g_mom_item<M, UNIT>(d.P, d.S, d.g, d.bw, blk);
+    p2g_mom_item<M, UNIT>(d.P, d.S, d.g, d.bw, j);
     __syncthreads();
   }
 }
 
 // Also empties the batch resolve list for the next step (the resolves of
-// this one have run).
+// this one have run).  Walks the order of the set the re-sort wrote.
 __global__ __launch_bounds__(256) void k_batch_g2p(const SceneDev *__restrict__ sd,
-                                                   const uint32_t *__restrict__ pre, uint32_t ns,
-                                                   uint32_t total, uint32_t *nlist) {
+                                                   const uint2 *__restrict__ order,
+                                                   const uint32_t *__restrict__ tot,
+                                                   uint32_t *nlist) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *nlist = 0u;
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const uint32_t si = find_scene(pre, ns, u);
+  const uint32_t nu = tot[kOrderTotal * kAccStride];
+  for (uint32_t u = blockIdx.x; u < nu; u += gridDim.x) {
+    const uint2 e = order[u];
+    const uint32_t si = __builtin_amdgcn_readfirstlane(e.x), j = __builtin_amdgcn_readfirstlane(e.y);
     const SceneDev &d = sd[si];
-    g2p_item<0, true>(d.P, d.S, d.D, d.g, d.W, d.bw, d.src_of, u - pre[si]);
+    g2p_item<0, true>(d.P, d.S, d.D, d.g, d.W, d.bw, d.src_of, j);
     __syncthreads();
+  }
+}
+
+// The dispatch order, three launches: the set's counters zeroed; per scene
+// (one block each, persistent) its items counted per row class (class
+// kMaxPpt - rows: rows are absolute, so scenes of different item sizes
+// interleave by size) and each class's range reserved on the totals; then
+// per scene every item placed at the class's offset (the larger classes'
+// totals) + the scene's base + its rank among the scene's items of that
+// class.  Order within a class is irrelevant to every result.
+__global__ void k_batch_order_zero(uint32_t *tot) {
+  if (threadIdx.x <= kOrderTotal) tot[threadIdx.x * kAccStride] = 0u;
+}
+
+__device__ __forceinline__ uint32_t item_class(uint32_t count) {
+  return (uint32_t)kMaxPpt - (count + kTB - 1u) / kTB;
+}
+
+template <bool NEXT>
+__global__ __launch_bounds__(256) void k_batch_order_count(const SceneDev *__restrict__ sd,
+                                                           uint32_t ns, uint32_t *tot,
+                                                           uint32_t *ocls) {
+  __shared__ uint32_t cnt[kMaxPpt];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t si = blockIdx.x; si < ns; si += gridDim.x) {
+    const BrickWork &bw = sd[si].bw;
+    const uint4 *items = NEXT ? bw.items_w : bw.items;
+    const uint32_t nit = *(NEXT ? bw.nitems_w : bw.nitems);
+    if (t < (uint32_t)kMaxPpt) cnt[t] = 0u;
+    __syncthreads();
+    for (uint32_t j = t; j < nit; j += blockDim.x) atomicAdd(&cnt[item_class(items[j].z)], 1u);
+    __syncthreads();
+    if (t < (uint32_t)kMaxPpt) {
+      const uint32_t c = cnt[t];
+      ocls[(size_t)si * kMaxPpt + t] = c ? atomicAdd(&tot[t * kAccStride], c) : 0u;
+    }
+    if (t == kOrderTotal && nit) atomicAdd(&tot[kOrderTotal * kAccStride], nit);
+    __syncthreads();
+  }
+}
+
+template <bool NEXT>
+__global__ __launch_bounds__(256) void k_batch_order_place(const SceneDev *__restrict__ sd,
+                                                           uint32_t ns,
+                                                           const uint32_t *__restrict__ tot,
+                                                           const uint32_t *__restrict__ ocls,
+                                                           uint2 *order) {
+  __shared__ uint32_t base[kMaxPpt];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t si = blockIdx.x; si < ns; si += gridDim.x) {
+    const BrickWork &bw = sd[si].bw;
+    const uint4 *items = NEXT ? bw.items_w : bw.items;
+    const uint32_t nit = *(NEXT ? bw.nitems_w : bw.nitems);
+    if (t < (uint32_t)kMaxPpt) {
+      uint32_t off = 0;
+      for (uint32_t k = 0; k < t; k++) off += tot[k * kAccStride];
+      base[t] = off + ocls[(size_t)si * kMaxPpt + t];
+    }
+    __syncthreads();
+    for (uint32_t j = t; j < nit; j += blockDim.x)
+      order[atomicAdd(&base[item_class(items[j].z)], 1u)] = make_uint2(si, j);
+    __syncthreads();
+  }
+}
+
+void launch_batch_order(const BatchTables &t, bool next, hipStream_t st) {
+  const uint32_t ns = t.ns;
+  if (!ns) return;
+  uint32_t *tot = next ? t.tot_next : const_cast<uint32_t *>(t.tot_cur);
+  uint2 *order = next ? t.order_next : const_cast<uint2 *>(t.order_cur);
+  const uint32_t g = std::min<uint32_t>(ns, 2048);
+  k_batch_order_zero<<<1, 64, 0, st>>>(tot);
+  if (next) {
+    k_batch_order_count<true><<<g, 256, 0, st>>>(t.sd, ns, tot, t.ocls);
+    k_batch_order_place<true><<<g, 256, 0, st>>>(t.sd, ns, tot, t.ocls, order);
+  } else {
+    k_batch_order_count<false><<<g, 256, 0, st>>>(t.sd, ns, tot, t.ocls);
+    k_batch_order_place<false><<<g, 256, 0, st>>>(t.sd, ns, tot, t.ocls, order);
   }
 }
 
@@ -156,10 +243,10 @@ int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &gr
   const uint32_t gp = grid_of(t.tot_pblocks, grid.aux);
 #define MPM_BATCH_ITEMS(K)                                                         \
   do {                                                                             \
-    if (margin == 1 && t.all_unit) K<1, true><<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items); \
-    else if (margin == 1) K<1, false><<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);         \
-    else if (t.all_unit) K<2, true><<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);           \
-    else K<2, false><<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items);                         \
+    if (margin == 1 && t.all_unit) K<1, true><<<gi, 256, 0, st>>>(t.sd, t.order_cur, t.tot_cur); \
+    else if (margin == 1) K<1, false><<<gi, 256, 0, st>>>(t.sd, t.order_cur, t.tot_cur);         \
+    else if (t.all_unit) K<2, true><<<gi, 256, 0, st>>>(t.sd, t.order_cur, t.tot_cur);           \
+    else K<2, false><<<gi, 256, 0, st>>>(t.sd, t.order_cur, t.tot_cur);                         \
   } while (0)
   MPM_BATCH_ITEMS(k_batch_mass);
   // the re-sort preparation needs only the mass pass: it overlaps the
@@ -169,6 +256,7 @@ int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &gr
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step fork");
   k_batch_scan_items<<<std::min<uint32_t>(ns, 2048), 256, 0, side>>>(t.sd, ns);
   k_batch_build_src<<<gp, 256, 0, side>>>(t.sd, t.pre_pblocks, ns, t.tot_pblocks);
+  launch_batch_order(t, true, side);
   e = hipEventRecord(join, side);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step join");
   k_batch_brick_list<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks, t.list, t.nlist);
@@ -178,7 +266,7 @@ int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &gr
   k_batch_resolve_update<<<gr, kBatchResolveThreads, 0, st>>>(t.sd, t.list, t.nlist);
   e = hipStreamWaitEvent(st, join, 0);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step wait");
-  k_batch_g2p<<<gi, 256, 0, st>>>(t.sd, t.pre_items, ns, t.tot_items, t.nlist);
+  k_batch_g2p<<<gi, 256, 0, st>>>(t.sd, t.order_next, t.tot_next, t.nlist);
   return MPM_OK;
 }
 
@@ -255,8 +343,18 @@ int fold_events(mpm_batch *b) {
 // Descriptor tables and unit prefixes of the current members.
 // Byte offset of the batch resolve list in the table allocation: the two
 // descriptor tables, the prefixes, then 256 bytes holding the list length.
+// After the list: the two dispatch orders (tot_items pairs each), their
+// counters and the per-scene class bases.
 static size_t list_off(size_t tbytes, size_t npre) {
   return ((2 * tbytes + npre * 4 + 255) & ~(size_t)255) + 256;
+}
+constexpr size_t kOrderCounters = (size_t)(kOrderTotal + 1) * kAccStride * 4;
+static_assert(kOrderTotal == (uint32_t)kMaxPpt, "one counter per row class, then the total");
+static size_t order_off(size_t tbytes, size_t npre, size_t nbricks) {
+  return (list_off(tbytes, npre) + nbricks * sizeof(uint2) + 255) & ~(size_t)255;
+}
+static size_t order_bytes(size_t nitems, size_t ns) {
+  return 2 * (nitems * sizeof(uint2) + kOrderCounters) + ns * kMaxPpt * 4 + 256;
 }
 
 int build_tables(mpm_batch *b) {
@@ -292,7 +390,7 @@ int build_tables(mpm_batch *b) {
                      memcmp(hall.data(), b->host_tab.data(), hall.size() * sizeof(SceneDev)) == 0);
   if (!same) {
     drop_graphs(b);
-    const size_t need = list_off(tbytes, pre.size()) + (size_t)pb[ns] * sizeof(uint2) + 256;
+    const size_t need = order_off(tbytes, pre.size(), pb[ns]) + order_bytes(pi[ns], ns);
     if (need > b->dev_bytes) {
       if (b->dev) BHIP(hipFree(b->dev));
       b->dev = nullptr;
@@ -308,6 +406,11 @@ int build_tables(mpm_batch *b) {
   uint32_t *dpre = (uint32_t *)(base + 2 * tbytes);
   uint32_t *dnlist = (uint32_t *)(base + list_off(tbytes, pre.size()) - 256);
   uint2 *dlist = (uint2 *)(base + list_off(tbytes, pre.size()));
+  char *ob = base + order_off(tbytes, pre.size(), pb[ns]);
+  uint2 *dorder[2] = {(uint2 *)ob, (uint2 *)(ob + (size_t)pi[ns] * sizeof(uint2))};
+  uint32_t *dtot[2] = {(uint32_t *)(ob + 2 * (size_t)pi[ns] * sizeof(uint2)),
+                       (uint32_t *)(ob + 2 * (size_t)pi[ns] * sizeof(uint2) + kOrderCounters)};
+  uint32_t *docls = (uint32_t *)(ob + 2 * ((size_t)pi[ns] * sizeof(uint2) + kOrderCounters));
   if (ns && !same) {
     BHIP(hipMemcpyAsync(d0, h0.data(), tbytes, hipMemcpyHostToDevice, b->stream));
     BHIP(hipMemcpyAsync(d1, h1.data(), tbytes, hipMemcpyHostToDevice, b->stream));
@@ -334,7 +437,16 @@ int build_tables(mpm_batch *b) {
     t.all_unit = all_unit;
     t.list = dlist;
     t.nlist = dnlist;
+    t.order_cur = dorder[ph];
+    t.tot_cur = dtot[ph];
+    t.order_next = dorder[ph ^ 1];
+    t.tot_next = dtot[ph ^ 1];
+    t.ocls = docls;
   }
+  // the members' current work items (from their own steps or an earlier
+  // batch) in dispatch order for the first batched step
+  launch_batch_order(b->tab[0], false, b->stream);
+  BHIP(hipGetLastError());
   b->seen.resize(b->mem.size());
   for (size_t i = 0; i < b->mem.size(); i++) b->seen[i] = b->mem[i]->version;
   b->phase = 0;
