@@ -230,20 +230,47 @@ __device__ __forceinline__ void gen_finish(int32_t *topp, const uint32_t *total,
 // ---------------------------------------------------------------------------
 // Diffuse advect (WGSL_DIFFUSE_ADVECT, shaders.js:929-996).  Kills mark a
 // bitmask by slot; a scan then pushes the killed slots in slot order.
+// sample_grid_velocity (shaders.js:889-911).  All 27 cells are loaded before
+// the sum, at indexes clamped into the grid, so the loads overlap; a cell
+// outside the grid then adds an exact zero, as the reference skips it (the sum
+// starts at +0 and can never become -0, so adding +0 leaves it unchanged).
 __device__ __forceinline__ F3 sample_grid_velocity(const Params &P, const GState &g, F3 pos) {
   PW pw;
   particle_weights(pos.x, pos.y, pos.z, pw);
-  F3 v = f3(0.0f, 0.0f, 0.0f);
+  const int gx = (int)P.gx, gy = (int)P.gy, gz = (int)P.gz;
+  int cx[3], cy[3], cz[3];
+  bool ox[3], oy[3], oz[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const int x = pw.b[0] + a, y = pw.b[1] + a, z = pw.b[2] + a;
+    ox[a] = x >= 0 && x < gx;
+    oy[a] = y >= 0 && y < gy;
+    oz[a] = z >= 0 && z < gz;
+    cx[a] = min(max(x, 0), gx - 1);
+    cy[a] = min(max(y, 0), gy - 1);
+    cz[a] = min(max(z, 0), gz - 1);
+  }
+  float4 c[27];
+#pragma unroll
   for (int k = 0; k < 3; k++)
+#pragma unroll
     for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+        c[(k * 3 + j) * 3 + i] = g.v[(cz[k] * gy + cy[j]) * gx + cx[i]];
+  F3 v = f3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
       for (int i = 0; i < 3; i++) {
-        int idx = cell_idx_checked(pw.b[0] + i, pw.b[1] + j, pw.b[2] + k, P);
-        if (idx < 0) continue;
-        float w = (pw.w[0][i] * pw.w[1][j]) * pw.w[2][k];
-        float4 c = g.v[idx];
-        v.x += w * c.x;
-        v.y += w * c.y;
-        v.z += w * c.z;
+        const bool in = ox[i] && oy[j] && oz[k];
+        const float w = (pw.w[0][i] * pw.w[1][j]) * pw.w[2][k];
+        const float4 q = c[(k * 3 + j) * 3 + i];
+        v.x += in ? w * q.x : 0.0f;
+        v.y += in ? w * q.y : 0.0f;
+        v.z += in ? w * q.z : 0.0f;
       }
   return v;
 }
@@ -279,16 +306,17 @@ __global__ __launch_bounds__(256) void k_advect(Params P, GState g, DRec *pool, 
   const float dt = P.dt, gr = P.gravity;
   F3 pos3 = f3(p.pos.x, p.pos.y, p.pos.z);
   bool kill = false;
+  // foam and bubbles sample the fluid's velocity (one inlined gather for both)
+  const F3 gv = (p.ptype == 1u || p.ptype == 2u) ? sample_grid_velocity(P, g, pos3)
+                                                 : f3(0.0f, 0.0f, 0.0f);
   if (p.ptype == 0u) {
     p.vel.y += gr * dt;
     p.pos = make_float4(pos3.x + dt * p.vel.x, pos3.y + dt * p.vel.y, pos3.z + dt * p.vel.z, 0.0f);
   } else if (p.ptype == 1u) {
-    F3 gv = sample_grid_velocity(P, g, pos3);
     p.pos = make_float4(pos3.x + dt * gv.x, pos3.y + dt * gv.y, pos3.z + dt * gv.z, 0.0f);
     p.lifetime -= dt;
     if (p.lifetime <= 0.0f) kill = true;
   } else if (p.ptype == 2u) {
-    F3 gv = sample_grid_velocity(P, g, pos3);
     p.vel.x += (dt * P.drag) * (gv.x - p.vel.x);
     p.vel.y += dt * (P.buoyancy * (-gr) + P.drag * (gv.y - p.vel.y));
     p.vel.z += (dt * P.drag) * (gv.z - p.vel.z);
