@@ -274,7 +274,7 @@ def serve_bench(args) -> dict:
     tb_batch = autotune.load_table(key="batch_ppt_by_class")
     tb_alone = autotune.load_table(key="alone_ppt_by_class")
     srv = mlsmpm.Server(hbm_budget=200 << 30, chunk_steps=args.serve_chunk,
-                        tile_ppt_by_class=tb_batch)
+                        tile_ppt_by_class=tb_batch, batch_flags=args.serve_batch_flags)
     ids = []
     t0 = time.perf_counter()
     for (grid, pos, vel, _), k, ta in zip(mix, steps, arrivals):
@@ -328,6 +328,7 @@ def serve_bench(args) -> dict:
         "value": busy_rate,
         "unit": "particle-steps/s",
         "wall_rate": st["particle_steps"] / wall,
+        "wall_seconds": wall,
         "plain_rate": plain_rate,
         "plain_sample": int(len(sample)),
         "vs_plain": busy_rate / plain_rate,
@@ -340,6 +341,10 @@ def serve_bench(args) -> dict:
                               "p99": float(np.percentile(lat, 99))},
         "batched_steps": st["batched_steps"],
         "chunks": st["chunks"],
+        # batched steps replayed from captured graphs (member sets that last
+        # more than one chunk), and the captures they needed
+        "graph_steps": st["graph_steps"],
+        "graph_captures": st["graph_captures"],
         # where the loop thread's time went, and how much of the busy time the
         # device spent in batched steps
         "loop_seconds": {k: st[k + "_seconds"] for k in ("enqueue", "admit", "wait", "retire",
@@ -458,6 +463,9 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-chunk", type=int, default=16)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-batch-flags", type=int, default=0,
+                    help="batch tuning flags of the server (0: graphs for lasting member sets, "
+                         "1: every step eager)")
     ap.add_argument("--no-handoff", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES each (the plain copy that "

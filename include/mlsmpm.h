@@ -317,6 +317,11 @@ int mpm_tile_stats(mpm_scene *s, uint64_t out[4]);
 typedef struct mpm_batch mpm_batch;
 
 #define MPM_BATCH_NO_GRAPH 1u  /* mpm_batch_tuning.flags: enqueue every step eagerly */
+/* mpm_batch_tuning.flags: capture graphs only for descriptor tables that an
+ * earlier mpm_batch_step call already stepped, so a member set that changes
+ * at every call (a serving loop admitting and retiring scenes) pays no
+ * capture; one that stays is captured at its second call. */
+#define MPM_BATCH_GRAPH_REUSED 2u
 
 /* Launch shapes of the batched kernels, as chosen by the autotuner
  * (mlsmpm.autotune): persistent blocks of the work-item kernels (P2G mass,
@@ -384,6 +389,11 @@ typedef struct mpm_server_config {
      * floor(log2(n)) / 2, as the autotuner (mlsmpm.autotune) tuned them for
      * batched steps. */
     uint8_t tile_ppt_by_class[16];
+    /* mpm_batch_tuning.flags of the server's batch (0 = MPM_BATCH_GRAPH_REUSED:
+     * a member set that lasts more than one chunk replays captured graphs;
+     * MPM_BATCH_NO_GRAPH: every step enqueued eagerly). */
+    uint32_t batch_flags;
+    uint32_t pad;
 } mpm_server_config;
 
 typedef struct mpm_server_stats {
@@ -404,6 +414,8 @@ typedef struct mpm_server_stats {
     double gpu_step_seconds;  /* sum of the batched steps' device times (HIP events) */
     uint64_t chunks;          /* loop iterations that stepped the batch */
     uint64_t admitted;        /* requests admitted */
+    uint64_t graph_steps;     /* batched steps replayed from captured graphs */
+    uint64_t graph_captures;  /* graphs captured */
 } mpm_server_stats;
 
 typedef struct mpm_request_info {

@@ -66,6 +66,10 @@ struct BatchTables {
   uint32_t *ocls;
 };
 
+// Batched steps replayed from captured graphs, and graphs captured, over the
+// batch's life.
+void batch_graph_counts(const mpm_batch *b, uint64_t *graph_steps, uint64_t *captures);
+
 // Count of the items of an order (its last counter).
 constexpr uint32_t kOrderTotal = 16u;  // = kMaxPpt
 

@@ -294,7 +294,15 @@ struct mpm_batch {
   std::vector<SceneDev> host_tab;   // uploaded descriptors, phase 0 then phase 1
   std::vector<uint32_t> host_pre;   // uploaded prefixes
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // by start phase
+  // Timed graphs record each step's start and end as event nodes, into one
+  // of two event sets (halves) that alternate between replays, so a replay
+  // is enqueued before the previous one's events are read.
+  hipGraphExec_t tgraph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [phase][half]
+  std::vector<hipEvent_t> gev;      // [half][step][start, end]
+  int gnext = 0, gpending = -1;     // next half; half of the unread replay
   uint32_t graph_len = 0;
+  uint32_t tables_used = 0;         // step calls since the tables were built
+  uint64_t graph_steps_run = 0, graph_captures = 0;
   bool timing = false;
   static constexpr uint32_t kEv = 128;  // batched steps per event pool
   hipEvent_t ev[2 * kEv] = {};
@@ -322,11 +330,34 @@ int bfail(int code, const char *fmt, ...) {
 
 uint32_t cdiv32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Read the step times of the pending timed replay (waits for it).
+int fold_graph_events(mpm_batch *b) {
+  if (b->gpending < 0) return MPM_OK;
+  const size_t base = (size_t)b->gpending * 2 * b->graph_len;
+  for (uint32_t k = 0; k < b->graph_len; k++) {
+    BHIP(hipEventSynchronize(b->gev[base + 2 * k + 1]));
+    float ms = 0.0f;
+    BHIP(hipEventElapsedTime(&ms, b->gev[base + 2 * k], b->gev[base + 2 * k + 1]));
+    b->step_ms.push_back(ms);
+  }
+  b->gpending = -1;
+  return MPM_OK;
+}
+
+// Releases the captured graphs.  Callers have drained the stream; the
+// pending timed replay's step times are read first.
 void drop_graphs(mpm_batch *b) {
+  (void)fold_graph_events(b);
   for (hipGraphExec_t &g : b->graph) {
     if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
   }
+  for (auto &ph : b->tgraph)
+    for (hipGraphExec_t &g : ph) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  b->gnext = 0;
 }
 
 int fold_events(mpm_batch *b) {
@@ -451,6 +482,7 @@ int build_tables(mpm_batch *b) {
   for (size_t i = 0; i < b->mem.size(); i++) b->seen[i] = b->mem[i]->version;
   b->phase = 0;
   b->dirty = false;
+  if (!same) b->tables_used = 0;
   return MPM_OK;
 }
 
@@ -475,27 +507,55 @@ uint32_t graph_len_of(const mpm_batch *b) {
   return L + (L & 1u);  // even: the buffer parity returns
 }
 
-int capture(mpm_batch *b, int ph) {
+// Events of the timed graphs: two halves of L steps, a start and an end each.
+int ensure_graph_events(mpm_batch *b, uint32_t L) {
+  const size_t need = (size_t)4 * L;
+  while (b->gev.size() < need) {
+    hipEvent_t e = nullptr;
+    BHIP(hipEventCreate(&e));
+    b->gev.push_back(e);
+  }
+  return MPM_OK;
+}
+
+// Capture L steps from start phase ph into graph[ph], or, timed, into
+// tgraph[ph][half] with each step's start and end as event-record nodes.
+int capture(mpm_batch *b, int ph, int half) {
   const uint32_t L = graph_len_of(b);
+  const bool timed = half >= 0;
+  if (timed) {
+    int rc = ensure_graph_events(b, L);
+    if (rc) return rc;
+  }
   BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = MPM_OK;
-  for (uint32_t i = 0; i < L && !rc; i++)
-    rc = launch_batch_step(b->tab[(ph + (int)i) & 1], b->margin, grid_of(b), b->stream, b->side,
-                           b->ev_fork, b->ev_join);
+  hipError_t er = hipSuccess;
+  for (uint32_t i = 0; i < L && !rc && er == hipSuccess; i++) {
+    hipEvent_t *e = timed ? &b->gev[((size_t)half * L + i) * 2] : nullptr;
+    if (e) er = hipEventRecordWithFlags(e[0], b->stream, hipEventRecordExternal);
+    if (er == hipSuccess)
+      rc = launch_batch_step(b->tab[(ph + (int)i) & 1], b->margin, grid_of(b), b->stream, b->side,
+                             b->ev_fork, b->ev_join);
+    if (e && !rc && er == hipSuccess)
+      er = hipEventRecordWithFlags(e[1], b->stream, hipEventRecordExternal);
+  }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (!rc && er != hipSuccess) rc = hip_fail(er, "hipEventRecordWithFlags (capture)");
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-  e = hipGraphInstantiate(&b->graph[ph], g, nullptr, nullptr, 0);
+  hipGraphExec_t &out = timed ? b->tgraph[ph][half] : b->graph[ph];
+  e = hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    b->graph[ph] = nullptr;
+    out = nullptr;
     return hip_fail(e, "hipGraphInstantiate");
   }
   b->graph_len = L;
+  b->graph_captures++;
   return MPM_OK;
 }
 
@@ -516,6 +576,10 @@ int remove_member(mpm_batch *b, mpm_scene *s) {
 
 namespace mpm {
 int batch_detach(mpm_scene *s) { return s->batch ? remove_member(s->batch, s) : MPM_OK; }
+void batch_graph_counts(const mpm_batch *b, uint64_t *graph_steps, uint64_t *captures) {
+  *graph_steps = b->graph_steps_run;
+  *captures = b->graph_captures;
+}
 }  // namespace mpm
 
 extern "C" {
@@ -560,6 +624,7 @@ int mpm_batch_destroy(mpm_batch *b) {
   if (b->dev) (void)hipFree(b->dev);
   for (hipEvent_t &e : b->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->gev) (void)hipEventDestroy(e);
   if (b->side) (void)hipStreamSynchronize(b->side);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);
@@ -650,25 +715,46 @@ int mpm_batch_step(mpm_batch *b, uint32_t nsteps) {
     advance(b, nsteps);
     return MPM_OK;
   }
-  const bool use_graph = !b->timing && !(b->tune.flags & MPM_BATCH_NO_GRAPH);
+  // With MPM_BATCH_GRAPH_REUSED, tables are captured only once an earlier
+  // call has stepped them: a member set that changes at every call is never
+  // captured, one that stays is captured at its second call.
+  const bool reused = !(b->tune.flags & MPM_BATCH_GRAPH_REUSED) || b->tables_used > 0;
+  const bool use_graph = !(b->tune.flags & MPM_BATCH_NO_GRAPH) && reused;
+  b->tables_used++;
   const uint32_t L = graph_len_of(b);
   while (nsteps > 0) {
     if (use_graph && nsteps >= L) {
-      if (!b->graph[b->phase] || b->graph_len != L) {
-        if (b->graph_len != L) drop_graphs(b);
-        int rc = capture(b, b->phase);
-        if (rc) return rc;
+      if (b->graph_len != L) {
+        BHIP(hipStreamSynchronize(b->stream));
+        drop_graphs(b);
       }
-      BHIP(hipGraphLaunch(b->graph[b->phase], b->stream));
+      if (b->timing) {
+        const int h = b->gnext;
+        int rc = MPM_OK;
+        if (!b->tgraph[b->phase][h] && (rc = capture(b, b->phase, h))) return rc;
+        BHIP(hipGraphLaunch(b->tgraph[b->phase][h], b->stream));
+        // what ran before this replay is read while it runs: eager steps, or
+        // the replay that used the other half (never both)
+        if ((rc = fold_events(b)) || (rc = fold_graph_events(b))) return rc;
+        b->gpending = h;
+        b->gnext = h ^ 1;
+      } else {
+        if (!b->graph[b->phase]) {
+          int rc = capture(b, b->phase, -1);
+          if (rc) return rc;
+        }
+        BHIP(hipGraphLaunch(b->graph[b->phase], b->stream));
+      }
+      b->graph_steps_run += L;
       advance(b, L);
       nsteps -= L;
       continue;
     }
     if (b->timing) {
-      if (b->ev_used == mpm_batch::kEv) {
-        int rc = fold_events(b);
-        if (rc) return rc;
-      }
+      // a pending timed replay ran before this step: its times come first
+      int rc = fold_graph_events(b);
+      if (rc) return rc;
+      if (b->ev_used == mpm_batch::kEv && (rc = fold_events(b))) return rc;
       BHIP(hipEventRecord(b->ev[2 * b->ev_used], b->stream));
     }
     int rc = launch_batch_step(b->tab[b->phase], b->margin, grid_of(b), b->stream, b->side,
@@ -698,6 +784,7 @@ int mpm_batch_enable_timing(mpm_batch *b, int enable) {
   BHIP(hipStreamSynchronize(b->stream));
   b->timing = enable != 0;
   b->ev_used = 0;
+  b->gpending = -1;
   b->step_ms.clear();
   return MPM_OK;
 }
@@ -705,7 +792,10 @@ int mpm_batch_enable_timing(mpm_batch *b, int enable) {
 int mpm_batch_step_times(mpm_batch *b, float *ms, uint32_t cap, uint32_t *count) {
   if (!b) return bfail(MPM_ERR_INVALID, "mpm_batch_step_times: batch is NULL");
   BHIP(hipSetDevice(b->device));
+  // at most one of the two is pending: eager steps read a pending replay
+  // first, a replay reads the eager steps before it
   int rc = fold_events(b);
+  if (!rc) rc = fold_graph_events(b);
   if (rc) return rc;
   const uint32_t n = (uint32_t)b->step_ms.size();
   if (count) *count = n;

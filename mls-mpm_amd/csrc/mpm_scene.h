@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mlsmpm.h"
@@ -9,8 +11,41 @@
 #include "mpm_kernels.h"
 #include "mpm_tiled.h"
 
+namespace mpm {
+// Device memory handed out from slabs that stay allocated until the arena is
+// destroyed.  A scene created over an arena takes its buffers from it and
+// gives them back when it is freed, so a loop that creates and frees scenes
+// calls hipMalloc only while the slabs grow, and never hipFree (which waits
+// for the whole device).  Blocks are 256-byte aligned; a freed block merges
+// with free neighbours of the same slab; an allocation takes the smallest
+// free block that holds it.
+class Arena {
+ public:
+  Arena(int device, size_t slab_bytes) : device_(device), slab_bytes_(slab_bytes) {}
+  ~Arena();
+  Arena(const Arena &) = delete;
+  Arena &operator=(const Arena &) = delete;
+  hipError_t alloc(size_t bytes, void **p);
+  void release(void *p);
+  size_t reserved() const { return reserved_; }
+
+ private:
+  struct Block {
+    size_t size;
+    uint32_t slab;
+  };
+  int device_;
+  size_t slab_bytes_;
+  size_t reserved_ = 0;
+  std::mutex mu_;
+  std::vector<char *> slabs_;
+  std::map<char *, Block> free_, used_;
+};
+}  // namespace mpm
+
 struct mpm_scene {
   mpm_config cfg{};
+  mpm::Arena *arena = nullptr;  // device memory source (nullptr: hipMalloc)
   hipStream_t stream = nullptr;
   uint32_t n = 0;        // active particles
   uint32_t G = 0;        // cells
@@ -105,7 +140,8 @@ namespace mpm {
 int set_error(int code, const char *msg);
 int hip_fail(hipError_t e, const char *what);
 int validate_config(const mpm_config &c);
-int scene_init(mpm_scene *s, const mpm_config &c);
+// arena: where the scene's device buffers come from (nullptr: hipMalloc).
+int scene_init(mpm_scene *s, const mpm_config &c, Arena *arena = nullptr);
 void scene_free(mpm_scene *s);
 int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, const float *C9,
                 const float *mass, bool device_ptrs);

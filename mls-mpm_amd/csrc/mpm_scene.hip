@@ -94,9 +94,61 @@ size_t scene_footprint(const mpm_config &c) {
   return b;
 }
 
+Arena::~Arena() {
+  (void)hipSetDevice(device_);
+  for (char *b : slabs_) (void)hipFree(b);
+}
+
+hipError_t Arena::alloc(size_t bytes, void **p) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> lk(mu_);
+  auto best = free_.end();
+  for (auto it = free_.begin(); it != free_.end(); ++it)
+    if (it->second.size >= bytes && (best == free_.end() || it->second.size < best->second.size))
+      best = it;
+  if (best == free_.end()) {
+    const size_t sz = bytes > slab_bytes_ ? bytes : slab_bytes_;
+    char *b = nullptr;
+    hipError_t e = hipMalloc((void **)&b, sz);
+    if (e != hipSuccess) return e;
+    slabs_.push_back(b);
+    reserved_ += sz;
+    best = free_.emplace(b, Block{sz, (uint32_t)(slabs_.size() - 1)}).first;
+  }
+  char *a = best->first;
+  const Block blk = best->second;
+  free_.erase(best);
+  if (blk.size > bytes) free_.emplace(a + bytes, Block{blk.size - bytes, blk.slab});
+  used_.emplace(a, Block{bytes, blk.slab});
+  *p = a;
+  return hipSuccess;
+}
+
+void Arena::release(void *p) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto u = used_.find((char *)p);
+  if (u == used_.end()) return;
+  char *a = u->first;
+  Block blk = u->second;
+  used_.erase(u);
+  auto next = free_.find(a + blk.size);
+  if (next != free_.end() && next->second.slab == blk.slab) {
+    blk.size += next->second.size;
+    free_.erase(next);
+  }
+  auto it = free_.emplace(a, blk).first;
+  if (it != free_.begin()) {
+    auto prev = std::prev(it);
+    if (prev->second.slab == blk.slab && prev->first + prev->second.size == a) {
+      prev->second.size += blk.size;
+      free_.erase(it);
+    }
+  }
+}
+
 static int dalloc(mpm_scene *s, void **p, size_t bytes) {
   if (bytes == 0) bytes = 256;
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = s->arena ? s->arena->alloc(bytes, p) : hipMalloc(p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
@@ -247,8 +299,9 @@ static int zero_grids(mpm_scene *s) {
   return MPM_OK;
 }
 
-int scene_init(mpm_scene *s, const mpm_config &c) {
+int scene_init(mpm_scene *s, const mpm_config &c, Arena *arena) {
   s->cfg = c;
+  s->arena = arena;
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev == 0)
@@ -351,8 +404,6 @@ int scene_init(mpm_scene *s, const mpm_config &c) {
     HIPCHK(hipMemsetAsync(s->d.genbits, 0, (size_t)nwg * 4, s->stream));
     HIPCHK(hipMemsetAsync(s->d.killbits, 0, (size_t)nwk * 4, s->stream));
   }
-  for (int i = 0; i < mpm_scene::kEvPerStep * mpm_scene::kEvSteps; i++)
-    HIPCHK(hipEventCreate(&s->ev[i]));
   int rc = reset_diffuse(s);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -375,9 +426,15 @@ void scene_free(mpm_scene *s) {
   }
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   s->copy_stream = nullptr;
-  for (int i = 0; i < s->nowned; i++) (void)hipFree(s->owned[i]);
+  for (int i = 0; i < s->nowned; i++) {
+    if (s->arena) s->arena->release(s->owned[i]);
+    else (void)hipFree(s->owned[i]);
+  }
   s->nowned = 0;
-  if (s->stage) (void)hipFree(s->stage);
+  if (s->stage) {
+    if (s->arena) s->arena->release(s->stage);
+    else (void)hipFree(s->stage);
+  }
   s->stage = nullptr;
   for (int i = 0; i < mpm_scene::kEvPerStep * mpm_scene::kEvSteps; i++)
     if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
@@ -387,10 +444,13 @@ void scene_free(mpm_scene *s) {
 
 static int ensure_stage(mpm_scene *s, size_t bytes) {
   if (s->stage_bytes >= bytes) return MPM_OK;
-  if (s->stage) HIPCHK(hipFree(s->stage));
+  if (s->stage) {
+    if (s->arena) s->arena->release(s->stage);
+    else HIPCHK(hipFree(s->stage));
+  }
   s->stage = nullptr;
   s->stage_bytes = 0;
-  hipError_t e = hipMalloc(&s->stage, bytes);
+  hipError_t e = s->arena ? s->arena->alloc(bytes, &s->stage) : hipMalloc(&s->stage, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return failf(MPM_ERR_NOMEM, "staging hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
@@ -1273,6 +1333,9 @@ int scene_set_walls(mpm_scene *s, const float *wallp) {
 int scene_enable_timing(mpm_scene *s, bool on) {
   HIPCHK(hipSetDevice(s->cfg.device));
   HIPCHK(hipStreamSynchronize(s->stream));
+  // the event pool is created at the first enable
+  for (int i = 0; on && i < mpm_scene::kEvPerStep * mpm_scene::kEvSteps; i++)
+    if (!s->ev[i]) HIPCHK(hipEventCreate(&s->ev[i]));
   s->timing = on;
   s->ev_used = 0;
   s->graph_pending = -1;

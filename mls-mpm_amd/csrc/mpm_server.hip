@@ -35,6 +35,8 @@ using Clock = std::chrono::steady_clock;
 
 // Drift margin of every served scene (the members of one batch share it).
 constexpr uint32_t kServedMargin = 2;
+// Arena slab: scenes larger than this take a slab of their own size.
+constexpr size_t kArenaSlab = (size_t)1 << 30;
 
 struct Request {
   uint64_t id = 0;
@@ -79,6 +81,8 @@ struct mpm_server {
   std::vector<mpm_step_record> step_rec;               // per batched step
   std::thread loop;
   mpm_batch *batch = nullptr;
+  // device memory of the resident scenes: slabs kept for the server's life
+  std::unique_ptr<Arena> arena;
   int loop_error = 0;
   std::string loop_msg;
 
@@ -87,11 +91,15 @@ struct mpm_server {
 
 namespace {
 
-// Create and load one request's scene and take its first step (the sort of
-// the caller's order) on the scene's own stream, while the batch runs
-// (loop thread).
+// Create and load one request's scene (its buffers from the server's arena)
+// and take its first step (the sort of the caller's order) on the scene's
+// own stream, while the batch runs (loop thread).
 int prepare(mpm_server *srv, Request *r) {
-  int rc = mpm_create(&r->cfg, &r->scene);
+  int rc = validate_config(r->cfg);
+  if (!rc) {
+    r->scene = new mpm_scene();
+    rc = scene_init(r->scene, r->cfg, srv->arena.get());
+  }
   if (!rc)
     rc = mpm_reset(r->scene, r->n, r->pos3.data(), r->vel3.empty() ? nullptr : r->vel3.data(),
                    r->C9.empty() ? nullptr : r->C9.data(),
@@ -145,7 +153,9 @@ int enroll(mpm_server *srv, Request *r) {
   return MPM_OK;
 }
 
-// Read the final state, release the scene (loop thread).
+// Read the final state and release the scene (loop thread).  The scene has
+// left the batch: its read runs on its own stream, beside the next chunk,
+// and its buffers go back to the arena.
 void retire(mpm_server *srv, Request *r, bool cancelled) {
   int rc = MPM_OK;
   std::string msg;
@@ -158,7 +168,6 @@ void retire(mpm_server *srv, Request *r, bool cancelled) {
                             r->density.data());
     if (rc) msg = mpm_last_error();
   }
-  (void)mpm_batch_remove(srv->batch, r->scene);
   mpm_destroy(r->scene);
   r->scene = nullptr;
   std::lock_guard<std::mutex> lk(srv->mu);
@@ -178,6 +187,11 @@ void retire(mpm_server *srv, Request *r, bool cancelled) {
   srv->cv_done.notify_all();
 }
 
+struct Leaving {
+  Request *r;
+  bool cancelled;
+};
+
 void run_loop(mpm_server *srv) {
   if (hipSetDevice(srv->cfg.device) != hipSuccess || mpm_batch_create(srv->cfg.device, &srv->batch)) {
     std::lock_guard<std::mutex> lk(srv->mu);
@@ -192,16 +206,30 @@ void run_loop(mpm_server *srv) {
   tune.item_blocks = srv->cfg.item_blocks;
   if (srv->cfg.resolve_blocks) tune.resolve_blocks = srv->cfg.resolve_blocks;
   if (srv->cfg.aux_blocks) tune.aux_blocks = srv->cfg.aux_blocks;
+  tune.flags = srv->cfg.batch_flags ? srv->cfg.batch_flags : MPM_BATCH_GRAPH_REUSED;
   mpm_batch_set_tuning(srv->batch, &tune);
   mpm_batch_enable_timing(srv->batch, 1);
   const uint32_t chunk = srv->cfg.chunk_steps ? srv->cfg.chunk_steps : 8;
   std::vector<float> ms;
+  // requests that left the batch after the last chunk, retired while the
+  // next one runs
+  std::vector<Leaving> leaving;
+  auto retire_leaving = [&]() {
+    const double t0 = srv->now();
+    for (const Leaving &l : leaving) retire(srv, l.r, l.cancelled);
+    const bool any = !leaving.empty();
+    leaving.clear();
+    if (any) {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      srv->stats.retire_seconds += srv->now() - t0;
+    }
+  };
   for (;;) {
     // Admissions, in arrival order, while the budget and slot limit allow.
     std::vector<Request *> admitting;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
-      if (srv->active.empty() && srv->queue.empty()) {
+      if (srv->active.empty() && srv->queue.empty() && leaving.empty()) {
         if (srv->stop) break;
         srv->cv_work.wait_for(lk, std::chrono::milliseconds(20));
         continue;
@@ -227,13 +255,17 @@ void run_loop(mpm_server *srv) {
       }
     }
     // One chunk of batched steps over every resident scene, enqueued first;
-    // the admitted scenes are created, loaded and sorted while it runs.
+    // the requests that finished in the last chunk are read back and
+    // released, and the admitted scenes created, loaded and sorted, while
+    // it runs.
     const bool stepping = !srv->active.empty();
     uint32_t c = chunk;
     for (Request *r : srv->active) c = std::min(c, r->steps - r->done);
     const double tb = srv->now();
     int rc = stepping ? mpm_batch_step(srv->batch, c) : MPM_OK;
     const double t_enq = srv->now();
+    retire_leaving();
+    const double t_pre = srv->now();
     std::vector<Request *> prepared;
     for (Request *r : admitting)
       if (prepare(srv, r) == MPM_OK) prepared.push_back(r);
@@ -250,7 +282,7 @@ void run_loop(mpm_server *srv) {
     {
       std::lock_guard<std::mutex> lk(srv->mu);
       srv->stats.enqueue_seconds += t_enq - tb;
-      srv->stats.admit_seconds += t_prep - t_enq;
+      srv->stats.admit_seconds += t_prep - t_pre;
       srv->stats.admitted += admitting.size();
     }
     if (!stepping) {
@@ -271,6 +303,7 @@ void run_loop(mpm_server *srv) {
       srv->stats.wait_seconds += t_sync - t_prep;
       srv->stats.chunks++;
       for (uint32_t i = 0; i < cnt && i < ms.size(); i++) srv->stats.gpu_step_seconds += ms[i] * 1e-3;
+      batch_graph_counts(srv->batch, &srv->stats.graph_steps, &srv->stats.graph_captures);
     }
     if (rc) {
       std::string msg = mpm_last_error();
@@ -297,23 +330,23 @@ void run_loop(mpm_server *srv) {
       join_prepared();
       continue;
     }
-    std::vector<Request *> finished, cancelled;
+    std::vector<Request *> rest;
     {
       std::lock_guard<std::mutex> lk(srv->mu);
-      uint64_t ps = 0;
+      uint64_t ps = 0, np = 0;
       for (Request *r : srv->active) {
+        np += r->n;
         r->done += c;
         ps += (uint64_t)r->n * c;
-        if (r->cancel) cancelled.push_back(r);
-        else if (r->done >= r->steps) finished.push_back(r);
+        if (r->cancel) leaving.push_back({r, true});
+        else if (r->done >= r->steps) leaving.push_back({r, false});
+        else rest.push_back(r);
       }
       srv->stats.particle_steps += ps;
       srv->stats.batched_steps += c;
       srv->stats.resident_steps += (uint64_t)srv->active.size() * c;
-      srv->stats.busy_seconds += srv->now() - tb;
+      srv->stats.busy_seconds += t_sync - tb;
       srv->step_ms.insert(srv->step_ms.end(), ms.begin(), ms.begin() + std::min<size_t>(cnt, ms.size()));
-      uint64_t np = 0;
-      for (Request *r : srv->active) np += r->n;
       for (uint32_t i = 0; i < cnt && i < ms.size(); i++) {
         mpm_step_record q{};
         q.ms = ms[i];
@@ -323,26 +356,16 @@ void run_loop(mpm_server *srv) {
         srv->step_rec.push_back(q);
       }
     }
-    const double t_ret = srv->now();
-    for (Request *r : finished) retire(srv, r, false);
-    for (Request *r : cancelled) retire(srv, r, true);
+    // the finished and cancelled leave the batch now (its stream is idle)
+    // and are retired while the next chunk runs
+    for (const Leaving &l : leaving) (void)mpm_batch_remove(srv->batch, l.r->scene);
     {
       std::lock_guard<std::mutex> lk(srv->mu);
-      srv->stats.retire_seconds += srv->now() - t_ret;
-    }
-    if (!finished.empty() || !cancelled.empty()) {
-      auto keep = [&](Request *r) {
-        return std::find(finished.begin(), finished.end(), r) == finished.end() &&
-               std::find(cancelled.begin(), cancelled.end(), r) == cancelled.end();
-      };
-      std::lock_guard<std::mutex> lk(srv->mu);
-      std::vector<Request *> rest;
-      for (Request *r : srv->active)
-        if (keep(r)) rest.push_back(r);
       srv->active.swap(rest);
     }
     join_prepared();
   }
+  retire_leaving();
   mpm_batch_destroy(srv->batch);
   srv->batch = nullptr;
 }
@@ -368,6 +391,7 @@ int mpm_server_create(const mpm_server_config *cfg, mpm_server **out) {
     return sfail(MPM_ERR_INVALID, "device %d out of range (%d devices)", cfg->device, ndev);
   mpm_server *srv = new mpm_server();
   srv->cfg = *cfg;
+  srv->arena = std::make_unique<Arena>(cfg->device, kArenaSlab);
   srv->t0 = Clock::now();
   srv->loop = std::thread(run_loop, srv);
   *out = srv;

@@ -11,9 +11,9 @@ from .capi import (Config, MpmError, MPM_FLAG_NO_GRAPH, MPM_FLAG_KEEP_GRID, MPM_
 from .scene import DREC_DTYPE, Scene
 from .batch import Batch
 from .server import Server
-from .capi import BatchTuning, MPM_BATCH_NO_GRAPH
+from .capi import BatchTuning, MPM_BATCH_NO_GRAPH, MPM_BATCH_GRAPH_REUSED
 
 __all__ = [
-    "Config", "MpmError", "Scene", "Batch", "Server", "BatchTuning", "MPM_BATCH_NO_GRAPH", "DREC_DTYPE", "default_config", "config_from_aspect", "load",
+    "Config", "MpmError", "Scene", "Batch", "Server", "BatchTuning", "MPM_BATCH_NO_GRAPH", "MPM_BATCH_GRAPH_REUSED", "DREC_DTYPE", "default_config", "config_from_aspect", "load",
     "lib_path", "MPM_FLAG_KEEP_GRID", "MPM_FLAG_NO_LDS", "MPM_FLAG_NO_GRAPH",
 ]

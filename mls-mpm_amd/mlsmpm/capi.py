@@ -75,6 +75,7 @@ class Config(C.Structure):
 
 
 MPM_BATCH_NO_GRAPH = 1
+MPM_BATCH_GRAPH_REUSED = 2
 
 
 class BatchTuning(C.Structure):
@@ -93,7 +94,8 @@ class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_active", C.c_uint32), ("hbm_budget", C.c_uint64),
                 ("chunk_steps", C.c_uint32), ("item_blocks", C.c_uint32),
                 ("resolve_blocks", C.c_uint32), ("aux_blocks", C.c_uint32),
-                ("tile_ppt_by_class", C.c_uint8 * 16)]
+                ("tile_ppt_by_class", C.c_uint8 * 16), ("batch_flags", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
@@ -107,7 +109,8 @@ class ServerStats(C.Structure):
                 ("enqueue_seconds", C.c_double), ("admit_seconds", C.c_double),
                 ("wait_seconds", C.c_double), ("retire_seconds", C.c_double),
                 ("enroll_seconds", C.c_double), ("gpu_step_seconds", C.c_double),
-                ("chunks", C.c_uint64), ("admitted", C.c_uint64)]
+                ("chunks", C.c_uint64), ("admitted", C.c_uint64),
+                ("graph_steps", C.c_uint64), ("graph_captures", C.c_uint64)]
 
 
 class StepRecord(C.Structure):

@@ -19,13 +19,17 @@ def _p(a):
 class Server:
     def __init__(self, hbm_budget: int, device: int = 0, max_active: int = 0,
                  chunk_steps: int = 8, item_blocks: int = 0, resolve_blocks: int = 0,
-                 aux_blocks: int = 0, tile_ppt_by_class: dict | None = None):
+                 aux_blocks: int = 0, tile_ppt_by_class: dict | None = None,
+                 batch_flags: int = 0):
         """tile_ppt_by_class: {particle-count class: work-item size} applied to
         requests that leave tile_ppt at 0 (autotune.size_class gives a
-        request's class; autotune.load_table reads a tuned table)."""
+        request's class; autotune.load_table reads a tuned table).
+        batch_flags: the batch's tuning flags (0: graphs replayed for member
+        sets that last more than one chunk; MPM_BATCH_NO_GRAPH: eager)."""
         cfg = ServerConfig(device=device, max_active=max_active, hbm_budget=int(hbm_budget),
                            chunk_steps=chunk_steps, item_blocks=item_blocks,
-                           resolve_blocks=resolve_blocks, aux_blocks=aux_blocks)
+                           resolve_blocks=resolve_blocks, aux_blocks=aux_blocks,
+                           batch_flags=batch_flags)
         for c, ppt in (tile_ppt_by_class or {}).items():
             cfg.tile_ppt_by_class[int(c)] = int(ppt)
         self._h = C.c_void_p()

@@ -146,6 +146,40 @@ def test_batch_timing_records_every_step():
         s.close()
 
 
+@pytest.mark.parametrize("flags", [0, mlsmpm.MPM_BATCH_GRAPH_REUSED])
+def test_batch_timed_graph_replays_match_alone(flags):
+    """Timed batches replay graphs that record each step's events as nodes:
+    every step is timed, in order, through replays mixed with eager steps and
+    a membership change, and every member keeps the bits of running alone."""
+    batched = [make_scene(sh)[0] for sh in SHAPES[:3]]
+    alone = [make_scene(sh)[0] for sh in SHAPES[:3]]
+    with mlsmpm.Batch(tuning={"graph_steps": 4, "flags": flags}) as b:
+        for s in batched:
+            b.add(s)
+        b.step(1)
+        b.enable_timing(True)
+        total = 0
+        for n in (9, 8, 3, 12, 4):  # 4-step timed replays and eager steps
+            b.step(n)
+            total += n
+        st = b.step_times()
+        assert len(st) == total and np.all(st > 0)
+        b.remove(batched[2])
+        b.step(8)  # a new table: eager with GRAPH_REUSED, else replays
+        b.step(8)
+        st2 = b.step_times()
+        assert len(st2) == total + 16 and np.all(st2 > 0)
+        np.testing.assert_array_equal(st2[:total], st)
+        b.sync()
+    alone[2].step(1 + total)
+    for s in alone[:2]:
+        s.step(1 + total + 16)
+    for i, (x, y) in enumerate(zip(batched, alone)):
+        assert_same(x, y, f"scene {i}")
+    for s in batched + alone:
+        s.close()
+
+
 def test_batch_rejects_unsupported_members():
     cfg = mlsmpm.default_config(24, 24, 24, 1000, sort_period=8)
     with mlsmpm.Scene(cfg) as s, mlsmpm.Batch() as b:

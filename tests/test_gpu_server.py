@@ -88,6 +88,37 @@ def test_same_request_alone_and_in_a_changing_mix():
     assert_same(solo, alone(grid, pos, vel, steps), "served vs plain")
 
 
+@pytest.mark.parametrize("flags", [0, mlsmpm.MPM_BATCH_NO_GRAPH])
+def test_graph_replays_and_eager_chunks_give_the_alone_results(flags):
+    """A member set that lasts several chunks replays captured graphs (the
+    default); forced eager steps replay none.  Either way every result equals
+    the scene stepped alone, also for scenes whose memory came back to the
+    server's arena from retired requests."""
+    reqs = []
+    with mlsmpm.Server(hbm_budget=64 * GB, chunk_steps=16, batch_flags=flags) as srv:
+        for i in range(3):
+            grid, pos, vel = scene_inputs(i)
+            cfg = mlsmpm.default_config(*grid, len(pos))
+            reqs.append((srv.submit(cfg, pos, 97, vel=vel), grid, pos, vel, 97))
+        for rid, *_ in reqs[:3]:
+            assert srv.wait(rid, 120.0) == capi.MPM_REQ_DONE
+        # later requests reuse the retired scenes' arena blocks
+        for i in (3, 4, 5):
+            grid, pos, vel = scene_inputs(i)
+            cfg = mlsmpm.default_config(*grid, len(pos))
+            reqs.append((srv.submit(cfg, pos, 40 + i, vel=vel), grid, pos, vel, 40 + i))
+        for rid, grid, pos, vel, steps in reqs:
+            assert srv.wait(rid, 120.0) == capi.MPM_REQ_DONE
+            assert_same(srv.result(rid), alone(grid, pos, vel, steps), f"request {rid}")
+        st = srv.stats()
+        assert st["completed"] == 6 and st["failed"] == 0
+        assert len(srv.step_times()) == st["batched_steps"]
+        if flags == mlsmpm.MPM_BATCH_NO_GRAPH:
+            assert st["graph_steps"] == 0 and st["graph_captures"] == 0
+        else:
+            assert st["graph_steps"] >= 32 and st["graph_captures"] >= 1
+
+
 def test_budget_admission_and_limits():
     grid, pos, vel = scene_inputs(3)
     cfg = mlsmpm.default_config(*grid, len(pos))
