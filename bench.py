@@ -19,7 +19,9 @@ Prints one JSON line:
   roofline         the dominant kernel's algorithmic bytes per launch over its
                    mean launch time, from HIP events recorded on the scene's
                    stream around that kernel alone in a second timed region of
-                   the same steps from the same reset, against the 8 TB/s HBM
+                   the same steps from the same reset (less the events' own
+                   cost, an empty interval between two marks of the same
+                   steps), against the 8 TB/s HBM
                    peak (active cells: the mean of ten samples over that run); `traffic` is
                    the HBM bytes per launch that rocprofv3 FETCH_SIZE +
                    WRITE_SIZE read for that kernel (profiles/traffic.json),
@@ -247,32 +249,15 @@ def batch_bench(args) -> dict:
     }
 
 
-def serve_bench(args) -> dict:
-    """Serving form: a seeded Poisson stream of requests (dam-break or
-    two-block scene, grid edges 32..128, 10k..2M particles log-uniform, steps
-    uniform in 100..2000) at `serve_rate` per second for `serve_seconds`,
-    admitted by an HBM budget of 200 GB into one continuously batched loop.
-    Reports aggregate particle-steps/s (over the loop's busy time and over the
-    wall time from first arrival to last completion), p50/p99 batched-step
-    latency and p50/p99 request latency; `vs_plain` divides the busy-time rate
-    by the same requests' plain single-scene rate (each scene's time
-    measured alone on a seeded sample of 48 requests: its full step count
-    from the initial state)."""
+def serve_stream(args, mix, steps, rate: float) -> dict:
+    """One seeded Poisson stream of the requests `mix` (with `steps` each) at
+    `rate` per second into a fresh server; waits for every result."""
     import mlsmpm
-    from mlsmpm import scenes
-
-    rng = np.random.default_rng(args.seed + 11)
-    nreq = max(1, int(round(args.serve_rate * args.serve_seconds)))
-    mix = scenes.scene_mix(args.seed + 5, nreq, kinds=("dam", "two"))
-    steps = rng.integers(100, 2001, nreq)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, nreq))
-    work = sum(len(p) * int(k) for (_, p, _, _), k in zip(mix, steps))
-    print(f"[bench] serve: {nreq} requests, {work:.3e} particle-steps", file=sys.stderr,
-          flush=True)
     from mlsmpm import autotune
 
+    nreq = len(mix)
+    arrivals = np.cumsum(np.random.default_rng(args.seed + 17).exponential(1.0 / rate, nreq))
     tb_batch = autotune.load_table(key="batch_ppt_by_class")
-    tb_alone = autotune.load_table(key="alone_ppt_by_class")
     srv = mlsmpm.Server(hbm_budget=200 << 30, chunk_steps=args.serve_chunk,
                         tile_ppt_by_class=tb_batch, batch_flags=args.serve_batch_flags)
     ids = []
@@ -293,6 +278,66 @@ def serve_bench(args) -> dict:
     step_ms = srv.step_times()
     rec = srv.step_records()
     srv.close()
+    return {
+        "rate_per_s": rate,
+        "arrival_seconds": float(arrivals[-1]),
+        "completed": st["completed"],
+        "particle_steps": st["particle_steps"],
+        "value": st["particle_steps"] / st["busy_seconds"],
+        "wall_rate": st["particle_steps"] / wall,
+        "wall_seconds": wall,
+        # the device's rate over its own batched-step time: what the loop
+        # reaches if its host work never leaves the device idle
+        "device_rate": st["particle_steps"] / max(st["gpu_step_seconds"], 1e-9),
+        "mean_resident": st["resident_steps"] / max(1, st["batched_steps"]),
+        "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                            "p99": float(np.percentile(step_ms, 99))},
+        "request_latency_s": {"p50": float(np.percentile(lat, 50)),
+                              "p99": float(np.percentile(lat, 99))},
+        "batched_steps": st["batched_steps"],
+        "chunks": st["chunks"],
+        # batched steps replayed from captured graphs (member sets that last
+        # more than one chunk), and the captures they needed
+        "graph_steps": st["graph_steps"],
+        "graph_captures": st["graph_captures"],
+        # where the loop thread's time went, and how much of the busy time the
+        # device spent in batched steps
+        "loop_seconds": {k: st[k + "_seconds"] for k in ("enqueue", "admit", "wait", "retire",
+                                                         "enroll")},
+        "gpu_step_seconds": st["gpu_step_seconds"],
+        "gpu_busy_frac": st["gpu_step_seconds"] / max(st["busy_seconds"], 1e-9),
+        "tail": tail_attribution(rec),
+    }
+
+
+def serve_bench(args) -> dict:
+    """Serving form: a seeded Poisson stream of requests (dam-break or
+    two-block scene, grid edges 32..128, 10k..2M particles log-uniform, steps
+    uniform in 100..2000), `serve_rate * serve_seconds` requests admitted by an
+    HBM budget of 200 GB into one continuously batched loop.  The stream runs
+    at `serve_rate` per second (the headline: latency at that load), then the
+    same requests at four times the rate (`saturated`: arrivals faster than
+    the loop finishes them, so the resident mix grows to what the budget
+    admits).  Each reports aggregate particle-steps/s (over the loop's busy
+    time, over the wall time from first arrival to last completion, and over
+    the device's batched-step time), p50/p99 batched-step latency and p50/p99
+    request latency; `vs_plain` divides the busy-time rate by the same
+    requests' plain single-scene rate (each scene's time measured alone on a
+    seeded sample of 48 requests: its full step count from the initial
+    state)."""
+    import mlsmpm
+    from mlsmpm import autotune, scenes
+
+    rng = np.random.default_rng(args.seed + 11)
+    nreq = max(1, int(round(args.serve_rate * args.serve_seconds)))
+    mix = scenes.scene_mix(args.seed + 5, nreq, kinds=("dam", "two"))
+    steps = rng.integers(100, 2001, nreq)
+    work = sum(len(p) * int(k) for (_, p, _, _), k in zip(mix, steps))
+    print(f"[bench] serve: {nreq} requests, {work:.3e} particle-steps", file=sys.stderr,
+          flush=True)
+    out = serve_stream(args, mix, steps, args.serve_rate)
+    sat = serve_stream(args, mix, steps, 4.0 * args.serve_rate)
+    tb_alone = autotune.load_table(key="alone_ppt_by_class")
     # plain form: a seeded sample of the requests, each scene alone running
     # its request's steps from the initial state (after a warm-up that
     # captures its graphs and a reset)
@@ -318,41 +363,17 @@ def serve_bench(args) -> dict:
 
     plain_rate = plain(None)
     plain_rate_t = plain(tb_alone)
-    busy_rate = st["particle_steps"] / st["busy_seconds"]
-    return {
-        "requests": nreq,
-        "completed": st["completed"],
-        "rate_per_s": args.serve_rate,
-        "hbm_budget_gb": 200,
-        "particle_steps": st["particle_steps"],
-        "value": busy_rate,
-        "unit": "particle-steps/s",
-        "wall_rate": st["particle_steps"] / wall,
-        "wall_seconds": wall,
-        "plain_rate": plain_rate,
-        "plain_sample": int(len(sample)),
-        "vs_plain": busy_rate / plain_rate,
-        "plain_rate_tuned": plain_rate_t,
-        "vs_plain_tuned": busy_rate / plain_rate_t,
-        "mean_resident": st["resident_steps"] / max(1, st["batched_steps"]),
-        "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)),
-                            "p99": float(np.percentile(step_ms, 99))},
-        "request_latency_s": {"p50": float(np.percentile(lat, 50)),
-                              "p99": float(np.percentile(lat, 99))},
-        "batched_steps": st["batched_steps"],
-        "chunks": st["chunks"],
-        # batched steps replayed from captured graphs (member sets that last
-        # more than one chunk), and the captures they needed
-        "graph_steps": st["graph_steps"],
-        "graph_captures": st["graph_captures"],
-        # where the loop thread's time went, and how much of the busy time the
-        # device spent in batched steps
-        "loop_seconds": {k: st[k + "_seconds"] for k in ("enqueue", "admit", "wait", "retire",
-                                                         "enroll")},
-        "gpu_step_seconds": st["gpu_step_seconds"],
-        "gpu_busy_frac": st["gpu_step_seconds"] / max(st["busy_seconds"], 1e-9),
-        "tail": tail_attribution(rec),
-    }
+    for o in (out, sat):
+        o["vs_plain"] = o["value"] / plain_rate
+        o["vs_plain_tuned"] = o["value"] / plain_rate_t
+        o["device_vs_plain"] = o["device_rate"] / plain_rate
+    res = {"requests": nreq, "hbm_budget_gb": 200, "unit": "particle-steps/s",
+           "offered_rate": work / float(out["arrival_seconds"]),
+           "plain_rate": plain_rate, "plain_sample": int(len(sample)),
+           "plain_rate_tuned": plain_rate_t}
+    res.update(out)
+    res["saturated"] = sat
+    return res
 
 
 def tail_attribution(rec) -> dict:
@@ -556,7 +577,11 @@ def main() -> int:
     dom = max(("g2p", "p2g_mom", "update"), key=lambda k: ms[k])
     bp, bc = ALG_BYTES[dom]
     alg = bp * n + bc * g_act
-    achieved = alg / (ms[dom] * 1e-3) if ms[dom] > 0 else 0.0
+    # The kernel's launch time: its event interval less the events' own cost,
+    # which the "relayout" interval measures (two marks with nothing between
+    # them, in the same captured steps).
+    launch_ms = ms[dom] - max(0.0, min(ms["relayout"], 0.5 * ms[dom]))
+    achieved = alg / (launch_ms * 1e-3) if launch_ms > 0 else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -572,8 +597,8 @@ def main() -> int:
     if os.path.exists(vpath):
         with open(vpath) as f:
             ve = json.load(f).get(args.workload, {}).get(dom)
-        if ve and ms[dom] > 0:
-            valu = {"floor_ms": ve["floor_ms"], "frac": ve["floor_ms"] / ms[dom],
+        if ve and launch_ms > 0:
+            valu = {"floor_ms": ve["floor_ms"], "frac": ve["floor_ms"] / launch_ms,
                     "insts_per_launch": ve["insts_valu"]}
     d_alive = s.diffuse_alive() if D else 0
     d_alive_mean = float(np.mean(a_samples)) if a_samples else float(d_alive)
@@ -616,7 +641,8 @@ def main() -> int:
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
             "alg_bytes_per_launch": alg,
-            "launch_ms": ms[dom],
+            "launch_ms": launch_ms,
+            "event_interval_ms": ms[dom],
             "valu": valu,
         },
         "step_roofline_frac": step_alg / (step_ms_mean * 1e-3) / HBM_PEAK,
