@@ -260,19 +260,43 @@ def serve_stream(args, mix, steps, rate: float) -> dict:
     tb_batch = autotune.load_table(key="batch_ppt_by_class")
     srv = mlsmpm.Server(hbm_budget=200 << 30, chunk_steps=args.serve_chunk,
                         tile_ppt_by_class=tb_batch, batch_flags=args.serve_batch_flags)
-    ids = []
+    # every request's client fetches its result as soon as it is done: a scan
+    # over the outstanding requests between arrivals and after the last one,
+    # not one wait after another in arrival order (which would hold the early
+    # finishers' results behind slower requests)
+    from mlsmpm import capi
+
+    final = (capi.MPM_REQ_DONE, capi.MPM_REQ_FAILED, capi.MPM_REQ_CANCELLED)
+    lat = []
+    pending = []
+
+    def drain():
+        nonlocal pending
+        rest = []
+        for rid in pending:
+            info = srv.poll(rid)
+            if info["state"] in final:
+                lat.append(info["t_done"] - info["t_submit"])
+                srv.result(rid)
+            else:
+                rest.append(rid)
+        done_any = len(rest) < len(pending)
+        pending = rest
+        return done_any
+
     t0 = time.perf_counter()
     for (grid, pos, vel, _), k, ta in zip(mix, steps, arrivals):
-        dt = ta - (time.perf_counter() - t0)
-        if dt > 0:
-            time.sleep(dt)
-        ids.append(srv.submit(mlsmpm.default_config(*grid, len(pos)), pos, int(k), vel=vel))
-    lat = []
-    for rid in ids:
-        srv.wait(rid, 600.0)
-        info = srv.poll(rid)
-        lat.append(info["t_done"] - info["t_submit"])
-        srv.result(rid)
+        while True:
+            dt = ta - (time.perf_counter() - t0)
+            if dt <= 0:
+                break
+            if not drain():
+                time.sleep(min(dt, 0.001))
+        pending.append(srv.submit(mlsmpm.default_config(*grid, len(pos)), pos, int(k), vel=vel))
+    deadline = time.perf_counter() + 600.0
+    while pending and time.perf_counter() < deadline:
+        if not drain():
+            time.sleep(0.001)
     wall = time.perf_counter() - t0
     st = srv.stats()
     step_ms = srv.step_times()
@@ -282,6 +306,7 @@ def serve_stream(args, mix, steps, rate: float) -> dict:
         "rate_per_s": rate,
         "arrival_seconds": float(arrivals[-1]),
         "completed": st["completed"],
+        "unfinished": len(pending),
         "particle_steps": st["particle_steps"],
         "value": st["particle_steps"] / st["busy_seconds"],
         "wall_rate": st["particle_steps"] / wall,

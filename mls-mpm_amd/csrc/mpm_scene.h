@@ -12,16 +12,18 @@
 #include "mpm_tiled.h"
 
 namespace mpm {
-// Device memory handed out from slabs that stay allocated until the arena is
-// destroyed.  A scene created over an arena takes its buffers from it and
-// gives them back when it is freed, so a loop that creates and frees scenes
-// calls hipMalloc only while the slabs grow, and never hipFree (which waits
-// for the whole device).  Blocks are 256-byte aligned; a freed block merges
-// with free neighbours of the same slab; an allocation takes the smallest
-// free block that holds it.
+// Memory handed out from slabs that stay allocated until the arena is
+// destroyed: device memory, or pinned host memory (`pinned`).  A scene
+// created over a device arena takes its buffers from it and gives them back
+// when it is freed, so a loop that creates and frees scenes calls hipMalloc
+// only while the slabs grow, and never hipFree (which waits for the whole
+// device).  Blocks are 256-byte aligned; a freed block merges with free
+// neighbours of the same slab; an allocation takes the smallest free block
+// that holds it.  A cap (0: none) bounds the slabs' total.
 class Arena {
  public:
-  Arena(int device, size_t slab_bytes) : device_(device), slab_bytes_(slab_bytes) {}
+  Arena(int device, size_t slab_bytes, bool pinned = false, size_t cap = 0)
+      : device_(device), slab_bytes_(slab_bytes), pinned_(pinned), cap_(cap) {}
   ~Arena();
   Arena(const Arena &) = delete;
   Arena &operator=(const Arena &) = delete;
@@ -36,6 +38,8 @@ class Arena {
   };
   int device_;
   size_t slab_bytes_;
+  bool pinned_;
+  size_t cap_;
   size_t reserved_ = 0;
   std::mutex mu_;
   std::vector<char *> slabs_;

@@ -96,7 +96,7 @@ size_t scene_footprint(const mpm_config &c) {
 
 Arena::~Arena() {
   (void)hipSetDevice(device_);
-  for (char *b : slabs_) (void)hipFree(b);
+  for (char *b : slabs_) (void)(pinned_ ? hipHostFree(b) : hipFree(b));
 }
 
 hipError_t Arena::alloc(size_t bytes, void **p) {
@@ -108,9 +108,14 @@ hipError_t Arena::alloc(size_t bytes, void **p) {
       best = it;
   if (best == free_.end()) {
     const size_t sz = bytes > slab_bytes_ ? bytes : slab_bytes_;
+    if (cap_ && reserved_ + sz > cap_) return hipErrorOutOfMemory;
     char *b = nullptr;
-    hipError_t e = hipMalloc((void **)&b, sz);
-    if (e != hipSuccess) return e;
+    hipError_t e = pinned_ ? hipHostMalloc((void **)&b, sz, hipHostMallocDefault)
+                           : hipMalloc((void **)&b, sz);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return e;
+    }
     slabs_.push_back(b);
     reserved_ += sz;
     best = free_.emplace(b, Block{sz, (uint32_t)(slabs_.size() - 1)}).first;

@@ -37,6 +37,10 @@ using Clock = std::chrono::steady_clock;
 constexpr uint32_t kServedMargin = 2;
 // Arena slab: scenes larger than this take a slab of their own size.
 constexpr size_t kArenaSlab = (size_t)1 << 30;
+// Pinned result arena: slab size and cap (results beyond it go to pageable
+// vectors until the clients fetch theirs).
+constexpr size_t kResultSlab = (size_t)256 << 20;
+constexpr size_t kResultCap = (size_t)16 << 30;
 
 struct Request {
   uint64_t id = 0;
@@ -48,6 +52,10 @@ struct Request {
   std::string error;
   size_t footprint = 0;
   mpm_scene *scene = nullptr;
+  // The final particles: a pinned block of the server's result arena (pos4,
+  // vel4, C9, density: 18 floats per particle), or, when that arena is full,
+  // these vectors.
+  float *res = nullptr;
   std::vector<float> pos4, vel4, C9out, density;
   double t_submit = 0, t_admit = 0, t_done = 0;
 };
@@ -83,6 +91,9 @@ struct mpm_server {
   mpm_batch *batch = nullptr;
   // device memory of the resident scenes: slabs kept for the server's life
   std::unique_ptr<Arena> arena;
+  // pinned host memory the finished requests' particles are read into (one
+  // fast copy on the loop thread), until their results are fetched
+  std::unique_ptr<Arena> results;
   int loop_error = 0;
   std::string loop_msg;
 
@@ -160,12 +171,23 @@ void retire(mpm_server *srv, Request *r, bool cancelled) {
   int rc = MPM_OK;
   std::string msg;
   if (!cancelled) {
-    r->pos4.resize((size_t)r->n * 4);
-    r->vel4.resize((size_t)r->n * 4);
-    r->C9out.resize((size_t)r->n * 9);
-    r->density.resize(r->n);
-    rc = mpm_read_particles(r->scene, r->pos4.data(), r->vel4.data(), r->C9out.data(),
-                            r->density.data());
+    const size_t n = r->n;
+    void *blk = nullptr;
+    if (n && srv->results->alloc(n * 18 * sizeof(float), &blk) == hipSuccess) {
+      r->res = (float *)blk;
+      rc = mpm_read_particles(r->scene, r->res, r->res + 4 * n, r->res + 8 * n, r->res + 17 * n);
+      if (rc) {
+        srv->results->release(blk);
+        r->res = nullptr;
+      }
+    } else {
+      r->pos4.resize(n * 4);
+      r->vel4.resize(n * 4);
+      r->C9out.resize(n * 9);
+      r->density.resize(n);
+      rc = mpm_read_particles(r->scene, r->pos4.data(), r->vel4.data(), r->C9out.data(),
+                              r->density.data());
+    }
     if (rc) msg = mpm_last_error();
   }
   mpm_destroy(r->scene);
@@ -392,6 +414,7 @@ int mpm_server_create(const mpm_server_config *cfg, mpm_server **out) {
   mpm_server *srv = new mpm_server();
   srv->cfg = *cfg;
   srv->arena = std::make_unique<Arena>(cfg->device, kArenaSlab);
+  srv->results = std::make_unique<Arena>(cfg->device, kResultSlab, true, kResultCap);
   srv->t0 = Clock::now();
   srv->loop = std::thread(run_loop, srv);
   *out = srv;
@@ -505,26 +528,40 @@ int mpm_server_wait(mpm_server *srv, uint64_t id, double timeout_s, int *state) 
 int mpm_server_result(mpm_server *srv, uint64_t id, float *pos4, float *vel4, float *C9,
                       float *density) {
   if (!srv) return sfail(MPM_ERR_INVALID, "mpm_server_result: server is NULL");
-  std::lock_guard<std::mutex> lk(srv->mu);
-  auto it = srv->reqs.find(id);
-  if (it == srv->reqs.end())
-    return sfail(MPM_ERR_NOT_FOUND, "request %llu unknown", (unsigned long long)id);
-  Request &r = *it->second;
-  if (r.state == MPM_REQ_FAILED) {
-    std::string msg = r.error;
+  std::unique_ptr<Request> r;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(id);
+    if (it == srv->reqs.end())
+      return sfail(MPM_ERR_NOT_FOUND, "request %llu unknown", (unsigned long long)id);
+    const int st = it->second->state;
+    if (st == MPM_REQ_FAILED) {
+      std::string msg = it->second->error;
+      srv->reqs.erase(it);
+      return sfail(MPM_ERR_STATE, "request failed: %s", msg.c_str());
+    }
+    if (st == MPM_REQ_CANCELLED) {
+      srv->reqs.erase(it);
+      return sfail(MPM_ERR_STATE, "request was cancelled");
+    }
+    if (st != MPM_REQ_DONE) return sfail(MPM_ERR_STATE, "request not finished");
+    // a finished request is the caller's alone: copied out after the lock
+    r = std::move(it->second);
     srv->reqs.erase(it);
-    return sfail(MPM_ERR_STATE, "request failed: %s", msg.c_str());
   }
-  if (r.state == MPM_REQ_CANCELLED) {
-    srv->reqs.erase(it);
-    return sfail(MPM_ERR_STATE, "request was cancelled");
+  if (r->res) {
+    const size_t n = r->n;
+    if (pos4) memcpy(pos4, r->res, n * 16);
+    if (vel4) memcpy(vel4, r->res + 4 * n, n * 16);
+    if (C9) memcpy(C9, r->res + 8 * n, n * 36);
+    if (density) memcpy(density, r->res + 17 * n, n * 4);
+    srv->results->release(r->res);
+  } else {
+    if (pos4) memcpy(pos4, r->pos4.data(), r->pos4.size() * 4);
+    if (vel4) memcpy(vel4, r->vel4.data(), r->vel4.size() * 4);
+    if (C9) memcpy(C9, r->C9out.data(), r->C9out.size() * 4);
+    if (density) memcpy(density, r->density.data(), r->density.size() * 4);
   }
-  if (r.state != MPM_REQ_DONE) return sfail(MPM_ERR_STATE, "request not finished");
-  if (pos4) memcpy(pos4, r.pos4.data(), r.pos4.size() * 4);
-  if (vel4) memcpy(vel4, r.vel4.data(), r.vel4.size() * 4);
-  if (C9) memcpy(C9, r.C9out.data(), r.C9out.size() * 4);
-  if (density) memcpy(density, r.density.data(), r.density.size() * 4);
-  srv->reqs.erase(it);
   return MPM_OK;
 }
 
