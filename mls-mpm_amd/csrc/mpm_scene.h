@@ -2,48 +2,40 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <map>
-#include <mutex>
 #include <vector>
 
 #include "../../include/mlsmpm.h"
+#include "mpm_arena.h"
 #include "mpm_batch.h"
 #include "mpm_kernels.h"
 #include "mpm_tiled.h"
 
 namespace mpm {
-// Memory handed out from slabs that stay allocated until the arena is
-// destroyed: device memory, or pinned host memory (`pinned`).  A scene
-// created over a device arena takes its buffers from it and gives them back
-// when it is freed, so a loop that creates and frees scenes calls hipMalloc
-// only while the slabs grow, and never hipFree (which waits for the whole
-// device).  Blocks are 256-byte aligned; a freed block merges with free
-// neighbours of the same slab; an allocation takes the smallest free block
-// that holds it.  A cap (0: none) bounds the slabs' total.
-class Arena {
+// Slabs of device memory, or of pinned host memory, for a SlabArena.
+struct HipSlabs {
+  int device;
+  bool pinned;
+  static constexpr int kNoMemory = (int)hipErrorOutOfMemory;
+  int get(size_t bytes, void **p) {
+    hipError_t e = pinned ? hipHostMalloc(p, bytes, hipHostMallocDefault) : hipMalloc(p, bytes);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return (int)e;
+  }
+  void put(void *p) {
+    (void)hipSetDevice(device);
+    (void)(pinned ? hipHostFree(p) : hipFree(p));
+  }
+};
+
+// Device (or pinned host) memory from slabs kept until the arena is
+// destroyed (mpm_arena.h).  A scene created over a device arena takes its
+// buffers from it and gives them back when it is freed, so a loop that
+// creates and frees scenes calls hipMalloc only while the slabs grow, and
+// never hipFree (which waits for the whole device).
+class Arena : public SlabArena<HipSlabs> {
  public:
   Arena(int device, size_t slab_bytes, bool pinned = false, size_t cap = 0)
-      : device_(device), slab_bytes_(slab_bytes), pinned_(pinned), cap_(cap) {}
-  ~Arena();
-  Arena(const Arena &) = delete;
-  Arena &operator=(const Arena &) = delete;
-  hipError_t alloc(size_t bytes, void **p);
-  void release(void *p);
-  size_t reserved() const { return reserved_; }
-
- private:
-  struct Block {
-    size_t size;
-    uint32_t slab;
-  };
-  int device_;
-  size_t slab_bytes_;
-  bool pinned_;
-  size_t cap_;
-  size_t reserved_ = 0;
-  std::mutex mu_;
-  std::vector<char *> slabs_;
-  std::map<char *, Block> free_, used_;
+      : SlabArena<HipSlabs>(HipSlabs{device, pinned}, slab_bytes, cap) {}
 };
 }  // namespace mpm
 

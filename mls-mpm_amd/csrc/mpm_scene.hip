@@ -94,73 +94,16 @@ size_t scene_footprint(const mpm_config &c) {
   return b;
 }
 
-Arena::~Arena() {
-  (void)hipSetDevice(device_);
-  for (char *b : slabs_) (void)(pinned_ ? hipHostFree(b) : hipFree(b));
-}
-
-hipError_t Arena::alloc(size_t bytes, void **p) {
-  bytes = (bytes + 255) & ~(size_t)255;
-  std::lock_guard<std::mutex> lk(mu_);
-  auto best = free_.end();
-  for (auto it = free_.begin(); it != free_.end(); ++it)
-    if (it->second.size >= bytes && (best == free_.end() || it->second.size < best->second.size))
-      best = it;
-  if (best == free_.end()) {
-    const size_t sz = bytes > slab_bytes_ ? bytes : slab_bytes_;
-    if (cap_ && reserved_ + sz > cap_) return hipErrorOutOfMemory;
-    char *b = nullptr;
-    hipError_t e = pinned_ ? hipHostMalloc((void **)&b, sz, hipHostMallocDefault)
-                           : hipMalloc((void **)&b, sz);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      return e;
-    }
-    slabs_.push_back(b);
-    reserved_ += sz;
-    best = free_.emplace(b, Block{sz, (uint32_t)(slabs_.size() - 1)}).first;
-  }
-  char *a = best->first;
-  const Block blk = best->second;
-  free_.erase(best);
-  if (blk.size > bytes) free_.emplace(a + bytes, Block{blk.size - bytes, blk.slab});
-  used_.emplace(a, Block{bytes, blk.slab});
-  *p = a;
-  return hipSuccess;
-}
-
-void Arena::release(void *p) {
-  std::lock_guard<std::mutex> lk(mu_);
-  auto u = used_.find((char *)p);
-  if (u == used_.end()) return;
-  char *a = u->first;
-  Block blk = u->second;
-  used_.erase(u);
-  auto next = free_.find(a + blk.size);
-  if (next != free_.end() && next->second.slab == blk.slab) {
-    blk.size += next->second.size;
-    free_.erase(next);
-  }
-  auto it = free_.emplace(a, blk).first;
-  if (it != free_.begin()) {
-    auto prev = std::prev(it);
-    if (prev->second.slab == blk.slab && prev->first + prev->second.size == a) {
-      prev->second.size += blk.size;
-      free_.erase(it);
-    }
-  }
-}
-
 static int dalloc(mpm_scene *s, void **p, size_t bytes) {
   if (bytes == 0) bytes = 256;
-  hipError_t e = s->arena ? s->arena->alloc(bytes, p) : hipMalloc(p, bytes);
+  if (s->nowned >= (int)(sizeof s->owned / sizeof s->owned[0]))
+    return failf(MPM_ERR_INVALID, "internal: too many allocations");
+  hipError_t e = s->arena ? (hipError_t)s->arena->alloc(bytes, p) : hipMalloc(p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
     return failf(MPM_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
   }
-  if (s->nowned >= (int)(sizeof s->owned / sizeof s->owned[0]))
-    return failf(MPM_ERR_INVALID, "internal: too many allocations");
   s->owned[s->nowned++] = *p;
   s->bytes += bytes;
   return MPM_OK;
@@ -455,7 +398,8 @@ static int ensure_stage(mpm_scene *s, size_t bytes) {
   }
   s->stage = nullptr;
   s->stage_bytes = 0;
-  hipError_t e = s->arena ? s->arena->alloc(bytes, &s->stage) : hipMalloc(&s->stage, bytes);
+  hipError_t e =
+      s->arena ? (hipError_t)s->arena->alloc(bytes, &s->stage) : hipMalloc(&s->stage, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return failf(MPM_ERR_NOMEM, "staging hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));

@@ -173,7 +173,7 @@ void retire(mpm_server *srv, Request *r, bool cancelled) {
   if (!cancelled) {
     const size_t n = r->n;
     void *blk = nullptr;
-    if (n && srv->results->alloc(n * 18 * sizeof(float), &blk) == hipSuccess) {
+    if (n && srv->results->alloc(n * 18 * sizeof(float), &blk) == 0) {
       r->res = (float *)blk;
       rc = mpm_read_particles(r->scene, r->res, r->res + 4 * n, r->res + 8 * n, r->res + 17 * n);
       if (rc) {
