@@ -327,8 +327,10 @@ def serve_stream(args, mix, steps, rate: float) -> dict:
         "graph_captures": st["graph_captures"],
         # where the loop thread's time went, and how much of the busy time the
         # device spent in batched steps
-        "loop_seconds": {k: st[k + "_seconds"] for k in ("enqueue", "admit", "wait", "retire",
-                                                         "enroll")},
+        "loop_seconds": {k: st[k + "_seconds"]
+                         for k in ("enqueue", "admit", "wait", "retire", "enroll",
+                                   "admit_create", "admit_load", "retire_read",
+                                   "retire_release")},
         "gpu_step_seconds": st["gpu_step_seconds"],
         "gpu_busy_frac": st["gpu_step_seconds"] / max(st["busy_seconds"], 1e-9),
         "tail": tail_attribution(rec),

@@ -416,6 +416,11 @@ typedef struct mpm_server_stats {
     uint64_t admitted;        /* requests admitted */
     uint64_t graph_steps;     /* batched steps replayed from captured graphs */
     uint64_t graph_captures;  /* graphs captured */
+    /* admit_seconds and retire_seconds split (seconds, loop thread): */
+    double admit_create_seconds;    /* creating admitted scenes (buffers, stream, zeroed state) */
+    double admit_load_seconds;      /* uploading their particles, enqueuing their first step */
+    double retire_read_seconds;     /* reading finished requests' particles back */
+    double retire_release_seconds;  /* releasing their scenes */
 } mpm_server_stats;
 
 typedef struct mpm_request_info {

@@ -66,6 +66,9 @@ struct BatchTables {
   uint32_t *ocls;
 };
 
+// The stream the batch's members run on.
+hipStream_t batch_stream(const mpm_batch *b);
+
 // Batched steps replayed from captured graphs, and graphs captured, over the
 // batch's life.
 void batch_graph_counts(const mpm_batch *b, uint64_t *graph_steps, uint64_t *captures);

@@ -576,6 +576,7 @@ int remove_member(mpm_batch *b, mpm_scene *s) {
 
 namespace mpm {
 int batch_detach(mpm_scene *s) { return s->batch ? remove_member(s->batch, s) : MPM_OK; }
+hipStream_t batch_stream(const mpm_batch *b) { return b->stream; }
 void batch_graph_counts(const mpm_batch *b, uint64_t *graph_steps, uint64_t *captures) {
   *graph_steps = b->graph_steps_run;
   *captures = b->graph_captures;

@@ -144,6 +144,8 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
 int scene_step(mpm_scene *s, uint32_t nsteps);
 int scene_prepare(mpm_scene *s);
 int scene_read_particles(mpm_scene *s, float *pos4, float *vel4, float *C9, float *density);
+int scene_export_async(mpm_scene *s, hipStream_t st, hipStream_t copy, void *dst,
+                       hipEvent_t exported, hipEvent_t done);
 int scene_read_grid(mpm_scene *s, int32_t *m, int32_t *mv3, float *mf, float *v3);
 int scene_read_diffuse(mpm_scene *s, void *rec48, uint32_t *free_list, int32_t *top);
 int scene_write_diffuse(mpm_scene *s, const void *rec48, const uint32_t *free_list, int32_t top);

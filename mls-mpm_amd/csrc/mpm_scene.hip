@@ -820,6 +820,29 @@ int scene_prepare(mpm_scene *s) {
   return rc;
 }
 
+// The particles in reset order as one block of 18 floats per particle (pos4,
+// vel4, C9, density planes), without waiting: the export kernel on stream
+// st, then one copy of the block into dst (host memory, pinned for the copy
+// to run beside other work) on stream copy, after event `exported`; event
+// `done` marks the copy's end.  The scene's buffers must outlive the copy.
+int scene_export_async(mpm_scene *s, hipStream_t st, hipStream_t copy, void *dst,
+                       hipEvent_t exported, hipEvent_t done) {
+  HIPCHK(hipSetDevice(s->cfg.device));
+  const size_t n = s->n;
+  const size_t need = n * 4 * (4 + 4 + 9 + 1);
+  int rc = ensure_stage(s, need ? need : 256);
+  if (rc) return rc;
+  char *stg = (char *)s->stage;
+  launch_export_particles(pstate(s, s->cur), (uint32_t)n, (float4 *)stg, (float4 *)(stg + n * 16),
+                          (float *)(stg + n * 32), (float *)(stg + n * 68), st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(exported, st));
+  HIPCHK(hipStreamWaitEvent(copy, exported, 0));
+  if (need) HIPCHK(hipMemcpyAsync(dst, stg, need, hipMemcpyDeviceToHost, copy));
+  HIPCHK(hipEventRecord(done, copy));
+  return MPM_OK;
+}
+
 int scene_read_particles(mpm_scene *s, float *pos4, float *vel4, float *C9, float *density) {
   HIPCHK(hipSetDevice(s->cfg.device));
   const uint32_t n = s->n;

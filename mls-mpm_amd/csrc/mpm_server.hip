@@ -106,21 +106,23 @@ namespace {
 // and take its first step (the sort of the caller's order) on the scene's
 // own stream, while the batch runs (loop thread).
 int prepare(mpm_server *srv, Request *r) {
+  const double t0 = srv->now();
   int rc = validate_config(r->cfg);
   if (!rc) {
     r->scene = new mpm_scene();
     rc = scene_init(r->scene, r->cfg, srv->arena.get());
   }
+  const double t1 = srv->now();
   if (!rc)
     rc = mpm_reset(r->scene, r->n, r->pos3.data(), r->vel3.empty() ? nullptr : r->vel3.data(),
                    r->C9.empty() ? nullptr : r->C9.data(),
                    r->mass.empty() ? nullptr : r->mass.data());
-  if (!rc && r->steps) {
-    rc = mpm_step(r->scene, 1);
-    if (!rc) {
-      std::lock_guard<std::mutex> lk(srv->mu);
-      r->done = 1;
-    }
+  if (!rc && r->steps) rc = mpm_step(r->scene, 1);
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (!rc && r->steps) r->done = 1;
+    srv->stats.admit_create_seconds += t1 - t0;
+    srv->stats.admit_load_seconds += srv->now() - t1;
   }
   if (rc) {
     std::string msg = mpm_last_error();
@@ -167,10 +169,20 @@ int enroll(mpm_server *srv, Request *r) {
 // Read the final state and release the scene (loop thread).  The scene has
 // left the batch: its read runs on its own stream, beside the next chunk,
 // and its buffers go back to the arena.
-void retire(mpm_server *srv, Request *r, bool cancelled) {
+void retire(mpm_server *srv, Request *r, bool cancelled, hipEvent_t copied = nullptr) {
+  const double t0 = srv->now();
   int rc = MPM_OK;
   std::string msg;
-  if (!cancelled) {
+  if (copied) {
+    // read back already: wait for its copy
+    const hipError_t e = hipEventSynchronize(copied);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "result copy");
+      msg = mpm_last_error();
+      srv->results->release(r->res);
+      r->res = nullptr;
+    }
+  } else if (!cancelled) {
     const size_t n = r->n;
     void *blk = nullptr;
     if (n && srv->results->alloc(n * 18 * sizeof(float), &blk) == 0) {
@@ -190,9 +202,12 @@ void retire(mpm_server *srv, Request *r, bool cancelled) {
     }
     if (rc) msg = mpm_last_error();
   }
+  const double t1 = srv->now();
   mpm_destroy(r->scene);
   r->scene = nullptr;
   std::lock_guard<std::mutex> lk(srv->mu);
+  srv->stats.retire_read_seconds += t1 - t0;
+  srv->stats.retire_release_seconds += srv->now() - t1;
   srv->bytes_resident -= r->footprint;
   r->t_done = srv->now();
   if (cancelled) {
@@ -209,9 +224,15 @@ void retire(mpm_server *srv, Request *r, bool cancelled) {
   srv->cv_done.notify_all();
 }
 
+// A request that left the batch after a chunk.  Its particles are exported
+// on the batch's stream before the next chunk and copied into its pinned
+// result block on the copy stream beside it (`issued`, events ev[0]: the
+// export, ev[1]: the copy); it is released once the copy is done.
 struct Leaving {
   Request *r;
   bool cancelled;
+  bool issued = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 void run_loop(mpm_server *srv) {
@@ -233,12 +254,50 @@ void run_loop(mpm_server *srv) {
   mpm_batch_enable_timing(srv->batch, 1);
   const uint32_t chunk = srv->cfg.chunk_steps ? srv->cfg.chunk_steps : 8;
   std::vector<float> ms;
+  // Finished requests' particles go to the host on their own stream, beside
+  // the batch's next chunk.
+  hipStream_t copy = nullptr;
+  if (hipStreamCreateWithFlags(&copy, hipStreamNonBlocking) != hipSuccess) copy = nullptr;
+  std::vector<hipEvent_t> evpool;
+  auto take_event = [&]() -> hipEvent_t {
+    hipEvent_t e = nullptr;
+    if (!evpool.empty()) {
+      e = evpool.back();
+      evpool.pop_back();
+    } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
+  };
   // requests that left the batch after the last chunk, retired while the
   // next one runs
   std::vector<Leaving> leaving;
+  // Export and copy of a leaving request's particles, enqueued before the
+  // next chunk (the batch's stream is idle: the export runs at once, the
+  // copy beside the chunk); left to retire's own read when it cannot be.
+  auto issue_read = [&](Leaving &l) {
+    const size_t n = l.r->n;
+    if (l.cancelled || !n || !copy) return;
+    void *blk = nullptr;
+    if (srv->results->alloc(n * 18 * sizeof(float), &blk) != 0) return;
+    l.ev[0] = take_event();
+    l.ev[1] = take_event();
+    if (l.ev[0] && l.ev[1] &&
+        scene_export_async(l.r->scene, batch_stream(srv->batch), copy, blk, l.ev[0], l.ev[1]) ==
+            MPM_OK) {
+      l.r->res = (float *)blk;
+      l.issued = true;
+    } else {
+      srv->results->release(blk);
+    }
+  };
   auto retire_leaving = [&]() {
     const double t0 = srv->now();
-    for (const Leaving &l : leaving) retire(srv, l.r, l.cancelled);
+    for (Leaving &l : leaving) {
+      retire(srv, l.r, l.cancelled, l.issued ? l.ev[1] : nullptr);
+      for (hipEvent_t e : l.ev)
+        if (e) evpool.push_back(e);
+    }
     const bool any = !leaving.empty();
     leaving.clear();
     if (any) {
@@ -378,9 +437,13 @@ void run_loop(mpm_server *srv) {
         srv->step_rec.push_back(q);
       }
     }
-    // the finished and cancelled leave the batch now (its stream is idle)
-    // and are retired while the next chunk runs
-    for (const Leaving &l : leaving) (void)mpm_batch_remove(srv->batch, l.r->scene);
+    // the finished and cancelled leave the batch now (its stream is idle),
+    // their particles are exported ahead of the next chunk, and they are
+    // retired while it runs
+    for (Leaving &l : leaving) {
+      (void)mpm_batch_remove(srv->batch, l.r->scene);
+      issue_read(l);
+    }
     {
       std::lock_guard<std::mutex> lk(srv->mu);
       srv->active.swap(rest);
@@ -388,6 +451,8 @@ void run_loop(mpm_server *srv) {
     join_prepared();
   }
   retire_leaving();
+  for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+  if (copy) (void)hipStreamDestroy(copy);
   mpm_batch_destroy(srv->batch);
   srv->batch = nullptr;
 }

@@ -110,7 +110,9 @@ class ServerStats(C.Structure):
                 ("wait_seconds", C.c_double), ("retire_seconds", C.c_double),
                 ("enroll_seconds", C.c_double), ("gpu_step_seconds", C.c_double),
                 ("chunks", C.c_uint64), ("admitted", C.c_uint64),
-                ("graph_steps", C.c_uint64), ("graph_captures", C.c_uint64)]
+                ("graph_steps", C.c_uint64), ("graph_captures", C.c_uint64),
+                ("admit_create_seconds", C.c_double), ("admit_load_seconds", C.c_double),
+                ("retire_read_seconds", C.c_double), ("retire_release_seconds", C.c_double)]
 
 
 class StepRecord(C.Structure):

@@ -118,7 +118,7 @@ def test_batch_and_server_fail_loudly_without_device():
 def test_abi_struct_sizes():
     assert C.sizeof(capi.BatchTuning) == 8 * 4
     assert C.sizeof(capi.ServerConfig) == 56
-    assert C.sizeof(capi.ServerStats) == 4 * 8 + 2 * 4 + 4 * 8 + 2 * 8 + 6 * 8 + 4 * 8
+    assert C.sizeof(capi.ServerStats) == 4 * 8 + 2 * 4 + 4 * 8 + 2 * 8 + 6 * 8 + 4 * 8 + 4 * 8
     assert C.sizeof(capi.RequestInfo) == 4 * 4 + 8 + 3 * 8 + 2 * 4
 
 
