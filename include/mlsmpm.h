@@ -103,7 +103,10 @@ int mpm_destroy(mpm_scene *s);
  * 278-281): load n particles.  pos3 is n*3 floats (required), vel3 n*3,
  * C9 n*9, mass n (NULL = zero, zero, 1.0).  Every coordinate must be finite
  * and in [1, grid - 1) so the 3x3x3 stencil stays inside the grid; otherwise
- * MPM_ERR_RANGE and the scene is unchanged. */
+ * MPM_ERR_RANGE and the scene is unchanged.  On the brick path (a sort
+ * period, no MPM_FLAG_NO_LDS) the particles are also sorted into bricks
+ * here, so the first step is an ordinary one; reads return them in the
+ * caller's order whatever the layout. */
 int mpm_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, const float *C9,
               const float *mass);
 
@@ -114,10 +117,11 @@ int mpm_step(mpm_scene *s, uint32_t nsteps);
 
 /* No counterpart in the reference (WebGPU compiles its pipelines at
  * creation): capture the HIP graphs that the next multi-step mpm_step will
- * replay, without running a step, so that call pays no capture.  Before the
- * first step after mpm_reset they are the graphs of the state that step
- * leaves.  A no-op where steps run eagerly (MPM_FLAG_NO_GRAPH, sort period
- * 0, no particles, batch members). */
+ * replay, without running a step, so that call pays no capture.  Where the
+ * next step must first sort the particles (after mpm_append_particles or
+ * mpm_delete_near) they are the graphs of the state that step leaves.  A
+ * no-op where steps run eagerly (MPM_FLAG_NO_GRAPH, sort period 0, no
+ * particles, batch members). */
 int mpm_prepare(mpm_scene *s);
 
 /* Wait for all work enqueued on the scene's stream. */
@@ -309,9 +313,9 @@ int mpm_tile_stats(mpm_scene *s, uint64_t out[4]);
  *     device; their work runs on the batch's stream while they are members
  *     (every single-scene call on a member stays valid and is ordered with
  *     the batch).
- *   - A member not yet stepped since its reset takes that first step (the
- *     sort of the caller's order) through the single-scene path, together
- *     with the other members.
+ *   - A member whose particles must first be sorted (after
+ *     mpm_append_particles or mpm_delete_near) takes that step through the
+ *     single-scene path, together with the other members.
  *   - mpm_destroy on a member removes it from its batch first.
  */
 typedef struct mpm_batch mpm_batch;

@@ -484,6 +484,17 @@ int scene_reset(mpm_scene *s, uint32_t n, const float *pos3, const float *vel3, 
   }
   int rc = zero_grids(s);
   if (rc) return rc;
+  // On the brick path the caller's order is sorted into bricks here, with
+  // the upload (after the brick flags were cleared: the sort writes the
+  // occupancy of its work-item set), so the first step is an ordinary one;
+  // the steps' kernels and results are those of sorting in the first step.
+  if (n && !(c.flags & MPM_FLAG_NO_LDS) && c.sort_period) {
+    const Params P = scene_params(s);
+    launch_sort(P, S, pstate(s, 1), s->sort, bw_of(s, 0), s->tc, s->stream);
+    HIPCHK(hipGetLastError());
+    s->cur = 1;
+    s->sorted_once = true;
+  }
   rc = reset_diffuse(s);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -558,10 +569,11 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   // whatever the brick flags say).
   const bool lds = !(s->cfg.flags & MPM_FLAG_NO_LDS) && period != 0;
   const bool tiled = lds && s->n;
-  // Layout policy: the first step after a reset sorts the caller-order
-  // particles with a full sort; afterwards every `period` steps the P2G mass
-  // pass records new bricks and G2P writes the new state in sorted order.
-  // The direct path has no key recording and sorts with the full sort.
+  // Layout policy: particles not sorted since they were loaded or changed
+  // (the direct path after a reset; appended or deleted particles) take the
+  // full sort first; afterwards every `period` steps the P2G mass pass
+  // records new bricks and G2P writes the new state in sorted order.  The
+  // direct path has no key recording and sorts with the full sort.
   bool relayout = false;
   if (period && s->n) {
     bool due = !s->sorted_once || s->since_sort >= period;
@@ -791,9 +803,10 @@ int scene_step(mpm_scene *s, uint32_t nsteps) {
 }
 
 // Capture, without running anything, the graphs that the next multi-step
-// call will replay: those of the current start state, or, before the first
-// step after a reset (which sorts the caller's order eagerly), those of the
-// state that step leaves (buffers swapped, one step since the sort).
+// call will replay: those of the current start state, or, when the next step
+// must first sort the particles (eagerly: after an append or a delete),
+// those of the state that step leaves (buffers swapped, one step since the
+// sort).
 int scene_prepare(mpm_scene *s) {
   const uint32_t period = s->cfg.sort_period;
   const bool tiled = !(s->cfg.flags & MPM_FLAG_NO_LDS) && period != 0;

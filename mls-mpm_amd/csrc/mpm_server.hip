@@ -103,7 +103,7 @@ struct mpm_server {
 namespace {
 
 // Create and load one request's scene (its buffers from the server's arena)
-// and take its first step (the sort of the caller's order) on the scene's
+// and take its first step (after the sort of the caller's order) on the scene's
 // own stream, while the batch runs (loop thread).
 int prepare(mpm_server *srv, Request *r) {
   const double t0 = srv->now();

@@ -17,7 +17,8 @@
 // over the per-brick counts gives brick offsets, the work items of the new
 // order and the inverse permutation; G2P then writes every particle's new
 // state straight into its new slot, so the re-sort moves no extra data.  Only
-// the first step after a reset (caller order) runs a separate scatter pass.
+// loading a scene (caller order), or changing its particles, runs a separate
+// scatter pass.
 //
 // P2G.  An item's block owns an LDS window: its brick, the 1-cell stencil
 // halo and a drift margin of M cells ((10+2M)^3 cells).  The block adds the
@@ -307,7 +308,8 @@ KeyOut key_out(const SortBufs &sb, const BrickWork &bw) {
   return ko;
 }
 
-// Full sort for the first step after a reset: brick keys, offsets, work items
+// Full sort of the caller's order (at a reset on the brick path; before the
+// next step after particles were appended or deleted): brick keys, offsets, work items
 // and a scatter of every particle into its slot.  Leaves the counts zeroed.
 void launch_sort(const Params &P, const PState &src, const PState &dst, const SortBufs &sb,
                  const BrickWork &bw, const TileCfg &tc, hipStream_t st) {
