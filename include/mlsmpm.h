@@ -411,7 +411,7 @@ typedef struct mpm_server_stats {
     double elapsed_seconds;   /* since mpm_server_create */
     /* Where the loop's time went (seconds, loop thread): */
     double enqueue_seconds;   /* enqueuing batched chunks (with descriptor table rebuilds) */
-    double admit_seconds;     /* creating, loading and first-stepping admitted scenes */
+    double admit_seconds;     /* creating and loading admitted scenes */
     double wait_seconds;      /* waiting for a chunk to finish on the device */
     double retire_seconds;    /* reading back and releasing finished requests */
     double enroll_seconds;    /* joining prepared scenes to the batch */
@@ -422,7 +422,7 @@ typedef struct mpm_server_stats {
     uint64_t graph_captures;  /* graphs captured */
     /* admit_seconds and retire_seconds split (seconds, loop thread): */
     double admit_create_seconds;    /* creating admitted scenes (buffers, stream, zeroed state) */
-    double admit_load_seconds;      /* uploading their particles, enqueuing their first step */
+    double admit_load_seconds;      /* uploading and sorting their particles */
     double retire_read_seconds;     /* reading finished requests' particles back */
     double retire_release_seconds;  /* releasing their scenes */
 } mpm_server_stats;

@@ -102,9 +102,9 @@ struct mpm_server {
 
 namespace {
 
-// Create and load one request's scene (its buffers from the server's arena)
-// and take its first step (after the sort of the caller's order) on the scene's
-// own stream, while the batch runs (loop thread).
+// Create and load one request's scene (its buffers from the server's arena;
+// the load sorts the caller's order into bricks on the scene's own stream),
+// while the batch runs (loop thread).  Every step then runs in the batch.
 int prepare(mpm_server *srv, Request *r) {
   const double t0 = srv->now();
   int rc = validate_config(r->cfg);
@@ -117,10 +117,8 @@ int prepare(mpm_server *srv, Request *r) {
     rc = mpm_reset(r->scene, r->n, r->pos3.data(), r->vel3.empty() ? nullptr : r->vel3.data(),
                    r->C9.empty() ? nullptr : r->C9.data(),
                    r->mass.empty() ? nullptr : r->mass.data());
-  if (!rc && r->steps) rc = mpm_step(r->scene, 1);
   {
     std::lock_guard<std::mutex> lk(srv->mu);
-    if (!rc && r->steps) r->done = 1;
     srv->stats.admit_create_seconds += t1 - t0;
     srv->stats.admit_load_seconds += srv->now() - t1;
   }
