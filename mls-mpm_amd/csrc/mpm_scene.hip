@@ -186,6 +186,7 @@ void scene_dev(const mpm_scene *s, int phase, SceneDev *d) {
   d->g = s->g;
   d->bw = bw_of(s, c);
   d->ko = key_out(s->sort, d->bw);
+  d->ko.cols = s->tc.ppt;
   d->offsets = s->sort.offsets;
   d->src_of = s->sort.src_of;
   d->ch = 256u * s->tc.ppt;
@@ -597,6 +598,7 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   if (relayout) {
     ko = key_out(s->sort, bw);
     ko.acc = s->sort.gacc;
+    ko.cols = s->tc.ppt;
   }
   if (tiled) launch_p2g_mass_tiled(P, S, s->g, s->tc, bw, ko, st);
   else launch_p2g_mass_global(P, S, s->g, st);

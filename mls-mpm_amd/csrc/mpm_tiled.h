@@ -68,6 +68,11 @@ struct SrcOut {
 struct KeyOut {
   uint32_t *counts, *keys, *rank;
   uint32_t *acc;  // set: the pass zeroes the re-sort scan's running totals (ScanArgs)
+  // Rows of a full work item (its particles / 256).  Above 1, the particles
+  // of a wavefront that stay in their brick take their block of consecutive
+  // ranks column by column (rank_in_block), so that the slots the lanes of
+  // one row read in G2P come from consecutive slots of the old order.
+  uint32_t cols;
 };
 
 bool tiled_ppt_supported(uint32_t ppt);

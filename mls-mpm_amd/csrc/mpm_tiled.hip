@@ -305,6 +305,7 @@ KeyOut key_out(const SortBufs &sb, const BrickWork &bw) {
   ko.keys = sb.keys;
   ko.rank = sb.rank;
   ko.acc = nullptr;
+  ko.cols = 0;
   return ko;
 }
 

@@ -124,11 +124,31 @@ __device__ __forceinline__ WinPos win_pos(const PW &pw, int ox, int oy, int oz) 
   return w;
 }
 
-// floor(r / d) for r < 2^13 and 1 <= d <= 16.  (r + 1/2) / d lies at least
-// 1/32 from every integer, and the float product is within 2^-21 * 2^13 of
-// it, so the truncation is exact (the reciprocal may be off by an ulp).
+// floor(r / d) for r < 2^13 and 1 <= d <= 16, or r < 2^7 and 1 <= d <= 64.
+// (r + 1/2) / d lies at least 1/(2d) from every integer (1/32, or 1/128),
+// and the float product is within 2^-21 * r of it (2^-8, or 2^-14), so the
+// truncation is exact (the reciprocal may be off by an ulp).
 __device__ __forceinline__ uint32_t div_small(uint32_t r, uint32_t d) {
   return (uint32_t)(((float)r + 0.5f) * __builtin_amdgcn_rcpf((float)d));
+}
+
+// Offset in a block of n consecutive ranks (n <= 64) of the block's p-th
+// particle, taken column by column: the offsets in order of (offset mod K,
+// offset), a bijection of [0, n).  Offsets K apart are the slots of
+// neighbouring threads in one row of an item (item_slot), so consecutive
+// particles of the block land in neighbouring threads of one row.
+__device__ __forceinline__ uint32_t rank_in_block(uint32_t p, uint32_t n, uint32_t K) {
+  const uint32_t q = div_small(n, K), rm = n - q * K, big = rm * (q + 1u);
+  uint32_t j, m;
+  if (p < big) {
+    j = div_small(p, q + 1u);
+    m = p - j * (q + 1u);
+  } else {  // q >= 1 here (p < n)
+    const uint32_t p2 = p - big, u = div_small(p2, q);
+    j = rm + u;
+    m = p2 - u * q;
+  }
+  return j + K * m;
 }
 
 // Slot of the particle of rank r among the c particles of an item: ranks are
@@ -236,7 +256,8 @@ __device__ __forceinline__ void p2g_mass_item(const Params &P, const PState &S,
         uint32_t base = 0;
         if ((int)lane == leader) base = atomicAdd(&lcnt[13], (uint32_t)__popcll(own));
         base = __shfl(base, leader, 64);
-        r = base + (uint32_t)__popcll(own & ((1ull << lane) - 1ull));
+        const uint32_t p = (uint32_t)__popcll(own & ((1ull << lane) - 1ull));
+        r = base + (ko.cols > 1u ? rank_in_block(p, (uint32_t)__popcll(own), ko.cols) : p);
       } else if (near) {
         r = atomicAdd(&lcnt[nb], 1u);
       } else {
