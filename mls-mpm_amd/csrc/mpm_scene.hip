@@ -609,7 +609,8 @@ static int enqueue_step(mpm_scene *s, hipEvent_t *e, bool graph_events = false) 
   if ((rc = mark(2))) return rc;
   if (tiled)
     launch_p2g_mom_tiled(P, S, s->g, s->tc, bw, st,
-                         relayout ? src_out(s->sort, s->tc) : SrcOut{});
+                         relayout ? src_out(s->sort, s->tc, pstate(s, s->cur ^ 1).density)
+                                  : SrcOut{});
   else launch_p2g_mom_global(P, S, s->g, st);
   if ((rc = mark(3))) return rc;
   if (tiled) launch_resolve_update(P, s->g, s->tc, bw, s->walls, keep, st);

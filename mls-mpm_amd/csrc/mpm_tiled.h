@@ -60,6 +60,9 @@ struct SrcOut {
   const uint32_t *keys, *rank;
   const uint2 *spans;
   uint32_t *src_of;
+  // set: the particle densities go to their new slots in this plane (the
+  // destination state's), and the re-sorting G2P leaves them there
+  float *dens_w;
   uint32_t chs;
 };
 
@@ -89,7 +92,7 @@ void launch_p2g_mass_tiled(const Params &P, const PState &S, const GState &g, co
 void launch_resolve_mass(const Params &P, const GState &g, const BrickWork &bw, hipStream_t st,
                          const ScanArgs &sa = ScanArgs{});
 ScanArgs scan_args(const SortBufs &sb, const TileCfg &tc);
-SrcOut src_out(const SortBufs &sb, const TileCfg &tc);
+SrcOut src_out(const SortBufs &sb, const TileCfg &tc, float *dens_w);
 void launch_clear_list(const BrickWork &bw, hipStream_t st);
 // so.src_of set: the pass also writes the re-sort's inverse permutation.
 void launch_p2g_mom_tiled(const Params &P, const PState &S, const GState &g, const TileCfg &tc,

@@ -71,7 +71,9 @@ template <int M, bool EXACT, bool GEN>
 __global__ __launch_bounds__(256) void k_g2p_brick(Params P, PState S, PState D, GState g,
                                                    Walls W, BrickWork bw, const uint32_t *src_of,
                                                    GenOut go) {
-  g2p_item<M, EXACT, GEN>(P, S, D, g, W, bw, src_of, blockIdx.x, go);
+  // a re-sort here follows a momentum pass that wrote the densities to
+  // their new slots
+  g2p_item<M, EXACT, GEN, EXACT>(P, S, D, g, W, bw, src_of, blockIdx.x, go);
 }
 
 // Offsets and work items of the new order, one block.
@@ -245,12 +247,13 @@ ScanArgs scan_args(const SortBufs &sb, const TileCfg &tc) {
   return sa;
 }
 
-SrcOut src_out(const SortBufs &sb, const TileCfg &tc) {
+SrcOut src_out(const SortBufs &sb, const TileCfg &tc, float *dens_w) {
   SrcOut so;
   so.keys = sb.keys;
   so.rank = sb.rank;
   so.spans = sb.spans;
   so.src_of = sb.src_of;
+  so.dens_w = dens_w;
   so.chs = log2_pow2(kTB * tc.ppt);
   return so;
 }

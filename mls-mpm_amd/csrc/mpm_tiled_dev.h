@@ -438,10 +438,14 @@ __device__ __forceinline__ void p2g_mom_item(const Params &P, const PState &S, c
             atomicAdd(&bw.oz[c], ez);
           });
     }
-    S.dens(s) = density;
     if (SRC) {
       const uint32_t ch = 1u << so.chs, istart = span.x + (cr & ~(ch - 1u));
-      so.src_of[istart + item_slot(cr & (ch - 1u), min(span.y - istart, ch))] = s;
+      const uint32_t d = istart + item_slot(cr & (ch - 1u), min(span.y - istart, ch));
+      so.src_of[d] = s;
+      if (so.dens_w) so.dens_w[d] = density;
+      else S.dens(s) = density;
+    } else {
+      S.dens(s) = density;
     }
     cur = nxt;
     ck = nk;
@@ -715,7 +719,9 @@ __device__ __forceinline__ void resolve_update_entry(const Params &P, const GSta
 // writes, on its new position and velocity, its density and the grid's
 // decoded mass (what the separate decide pass would read after it), marking
 // the spawners in go.
-template <int M, bool EXACT, bool GEN = false>
+// DMOVED (with EXACT): the momentum pass already wrote every density to its
+// new slot of D, so the re-sort does not move it.
+template <int M, bool EXACT, bool GEN = false, bool DMOVED = false>
 __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const PState &D,
                                          const GState &g, const Walls &W, const BrickWork &bw,
                                          const uint32_t *src_of, uint32_t blk,
@@ -790,7 +796,7 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
     in.id = __float_as_uint(q.w);
     if (EXACT) {
       in.m = P.unit_mass ? 1.0f : S.at(PM, in.s);  // unit mass: the plane is never read
-      in.dens = S.dens(in.s);
+      in.dens = DMOVED ? 0.0f : S.dens(in.s);
     }
   };
   auto body = [&](uint32_t i, const In &cur) {
@@ -816,10 +822,10 @@ __device__ __forceinline__ void g2p_item(const Params &P, const PState &S, const
     store_g2p(D, d, o, cur.id);
     if (EXACT) {
       if (!P.unit_mass) D.at(PM, d) = cur.m;
-      D.dens(d) = cur.dens;
+      if (!DMOVED) D.dens(d) = cur.dens;
     }
     if (GEN) {
-      const float dens = EXACT ? cur.dens : S.dens(cur.s);
+      const float dens = EXACT ? (DMOVED ? D.dens(d) : cur.dens) : S.dens(cur.s);
       const float m = P.unit_mass ? 1.0f : (EXACT ? cur.m : S.at(PM, cur.s));
       if (dens > 0.0f && dens <= P.density_bubble_min) {
         const uint32_t id = cur.id;
