@@ -22,6 +22,8 @@ struct SceneDev {
   KeyOut ko;           // brick counts, keys and ranks recorded by P2G mass
   uint32_t *offsets;   // per brick rank: first particle of the new order
   uint32_t *src_of;    // per new slot: old slot
+  uint2 *spans;        // per brick rank: [first, end) slots of the new order
+  SrcOut so;           // the momentum pass's re-sort outputs (src_of, densities)
   uint32_t ch;         // work-item size in particles (256 * ppt)
   int32_t keep;        // keep the i32 momentum grids readable
   Walls W;
@@ -46,9 +48,8 @@ struct BatchTables {
   const uint32_t *pre_items;    // work items (upper bound per scene)
   const uint32_t *pre_bricks;   // bricks (upper bound of the resolve list)
   const uint32_t *pre_bblocks;  // 256-thread blocks over bricks
-  const uint32_t *pre_pblocks;  // 256-thread blocks over particles
   uint32_t ns;
-  uint32_t tot_items, tot_bricks, tot_bblocks, tot_pblocks;
+  uint32_t tot_items, tot_bricks, tot_bblocks;
   uint32_t all_unit;            // every scene has unit particle mass
   // The step's resolve list over all scenes, (scene, brick) pairs
   // (capacity tot_bricks), and its length; empty at every step start.

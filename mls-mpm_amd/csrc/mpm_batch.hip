@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_batch_mom(const SceneDev *__restrict__ 
     const uint2 e = order[u];
     const uint32_t si = __builtin_amdgcn_readfirstlane(e.x), j = __builtin_amdgcn_readfirstlane(e.y);
     const SceneDev &d = sd[si];
-    p2g_mom_item<M, UNIT>(d.P, d.S, d.g, d.bw, j);
+    p2g_mom_item<M, UNIT, true>(d.P, d.S, d.g, d.bw, j, d.so);
     __syncthreads();
   }
 }
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_batch_g2p(const SceneDev *__restrict__ 
     const uint2 e = order[u];
     const uint32_t si = __builtin_amdgcn_readfirstlane(e.x), j = __builtin_amdgcn_readfirstlane(e.y);
     const SceneDev &d = sd[si];
-    g2p_item<0, true>(d.P, d.S, d.D, d.g, d.W, d.bw, d.src_of, j);
+    g2p_item<0, true, false, true>(d.P, d.S, d.D, d.g, d.W, d.bw, d.src_of, j);
     __syncthreads();
   }
 }
@@ -213,19 +213,8 @@ __global__ __launch_bounds__(256) void k_batch_scan_items(const SceneDev *__rest
                                                           uint32_t ns) {
   for (uint32_t si = blockIdx.x; si < ns; si += gridDim.x) {
     const SceneDev &d = sd[si];
-    scan_items_block<16>(d.bw, (uint32_t)__builtin_ctz(d.ch), d.ko.counts, d.offsets);
+    scan_items_block<16>(d.bw, (uint32_t)__builtin_ctz(d.ch), d.ko.counts, d.offsets, d.spans);
     __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(256) void k_batch_build_src(const SceneDev *__restrict__ sd,
-                                                         const uint32_t *__restrict__ pre,
-                                                         uint32_t ns, uint32_t total) {
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const uint32_t si = find_scene(pre, ns, u);
-    const SceneDev &d = sd[si];
-    build_src_one(d.P.n, d.bw.nbricks, d.ch, d.ko.keys, d.ko.rank, d.offsets, d.src_of,
-                  (u - pre[si]) * 256u + threadIdx.x);
   }
 }
 
@@ -240,7 +229,6 @@ int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &gr
   const uint32_t gi = grid_of(t.tot_items, grid.items);
   const uint32_t gr = grid_of(t.tot_bricks, grid.resolve);
   const uint32_t gb = grid_of(t.tot_bblocks, grid.aux);
-  const uint32_t gp = grid_of(t.tot_pblocks, grid.aux);
 #define MPM_BATCH_ITEMS(K)                                                         \
   do {                                                                             \
     if (margin == 1 && t.all_unit) K<1, true><<<gi, 256, 0, st>>>(t.sd, t.order_cur, t.tot_cur); \
@@ -249,23 +237,24 @@ int launch_batch_step(const BatchTables &t, uint32_t margin, const BatchGrid &gr
     else K<2, false><<<gi, 256, 0, st>>>(t.sd, t.order_cur, t.tot_cur);                         \
   } while (0)
   MPM_BATCH_ITEMS(k_batch_mass);
-  // the re-sort preparation needs only the mass pass: it overlaps the
-  // resolves and P2G momentum on the side stream
+  // The re-sort's scan (brick spans of the new order, its work items and
+  // their dispatch order) needs only the mass pass: it runs on the side
+  // stream beside the brick list and the mass resolve.  P2G momentum then
+  // writes the inverse permutation and the densities at their new slots.
   hipError_t e = hipEventRecord(fork, st);
   if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step fork");
   k_batch_scan_items<<<std::min<uint32_t>(ns, 2048), 256, 0, side>>>(t.sd, ns);
-  k_batch_build_src<<<gp, 256, 0, side>>>(t.sd, t.pre_pblocks, ns, t.tot_pblocks);
   launch_batch_order(t, true, side);
   e = hipEventRecord(join, side);
   if (e != hipSuccess) return hip_fail(e, "launch_batch_step join");
   k_batch_brick_list<<<gb, 256, 0, st>>>(t.sd, t.pre_bblocks, ns, t.tot_bblocks, t.list, t.nlist);
   k_batch_resolve_mass<<<gr, kBatchResolveThreads, 0, st>>>(t.sd, t.list, t.nlist);
+  e = hipStreamWaitEvent(st, join, 0);
+  if (e != hipSuccess) return hip_fail(e, "launch_batch_step wait");
   MPM_BATCH_ITEMS(k_batch_mom);
 #undef MPM_BATCH_ITEMS
   k_batch_resolve_update<<<gr, kBatchResolveThreads, 0, st>>>(t.sd, t.list, t.nlist);
-  e = hipStreamWaitEvent(st, join, 0);
-  if (e != hipSuccess) return hip_fail(e, "launch_batch_step wait");
   k_batch_g2p<<<gi, 256, 0, st>>>(t.sd, t.order_next, t.tot_next, t.nlist);
   return MPM_OK;
 }
@@ -289,6 +278,7 @@ struct mpm_batch {
   int phase = 0;                    // parity of batched steps since the tables were built
   std::vector<mpm_scene *> active;  // members with particles, in table order
   void *dev = nullptr;              // descriptor tables and prefixes
+  std::vector<void *> retired;      // replaced table buffers, freed with the batch
   size_t dev_bytes = 0;
   BatchTables tab[2]{};
   std::vector<SceneDev> host_tab;   // uploaded descriptors, phase 0 then phase 1
@@ -395,9 +385,9 @@ int build_tables(mpm_batch *b) {
     if (s->n) b->active.push_back(s);
   const uint32_t ns = (uint32_t)b->active.size();
   std::vector<SceneDev> h0(ns), h1(ns);
-  std::vector<uint32_t> pre(4 * (size_t)(ns + 1), 0u);
+  std::vector<uint32_t> pre(3 * (size_t)(ns + 1), 0u);
   uint32_t all_unit = 1;
-  uint32_t *pi = pre.data(), *pb = pi + (ns + 1), *pbb = pb + (ns + 1), *ppb = pbb + (ns + 1);
+  uint32_t *pi = pre.data(), *pb = pi + (ns + 1), *pbb = pb + (ns + 1);
   for (uint32_t i = 0; i < ns; i++) {
     mpm_scene *s = b->active[i];
     scene_dev(s, 0, &h0[i]);
@@ -407,7 +397,6 @@ int build_tables(mpm_batch *b) {
     pi[i + 1] = pi[i] + max_items(s->n, nb, s->tc.ppt);
     pb[i + 1] = pb[i] + nb;
     pbb[i + 1] = pbb[i] + cdiv32(nb, 256);
-    ppb[i + 1] = ppb[i] + cdiv32(s->n, 256);
   }
   const size_t tbytes = (size_t)ns * sizeof(SceneDev);
   std::vector<SceneDev> hall(h0);
@@ -423,13 +412,18 @@ int build_tables(mpm_batch *b) {
     drop_graphs(b);
     const size_t need = order_off(tbytes, pre.size(), pb[ns]) + order_bytes(pi[ns], ns);
     if (need > b->dev_bytes) {
-      if (b->dev) BHIP(hipFree(b->dev));
+      // The replaced buffer is freed with the batch, not here: hipFree
+      // synchronizes the whole device, and a stream capture that another
+      // thread runs meanwhile (a scene's graph, say) fails with it.  A
+      // quarter of headroom keeps such replacements rare.
+      if (b->dev) b->retired.push_back(b->dev);
       b->dev = nullptr;
       b->dev_bytes = 0;
       b->host_tab.clear();
       b->host_pre.clear();
-      BHIP(hipMalloc(&b->dev, need));
-      b->dev_bytes = need;
+      const size_t cap = need + need / 4;
+      BHIP(hipMalloc(&b->dev, cap));
+      b->dev_bytes = cap;
     }
   }
   char *base = (char *)b->dev;
@@ -459,12 +453,10 @@ int build_tables(mpm_batch *b) {
     t.pre_items = dpre;
     t.pre_bricks = dpre + (ns + 1);
     t.pre_bblocks = dpre + 2 * (ns + 1);
-    t.pre_pblocks = dpre + 3 * (ns + 1);
     t.ns = ns;
     t.tot_items = pi[ns];
     t.tot_bricks = pb[ns];
     t.tot_bblocks = pbb[ns];
-    t.tot_pblocks = ppb[ns];
     t.all_unit = all_unit;
     t.list = dlist;
     t.nlist = dnlist;
@@ -623,6 +615,7 @@ int mpm_batch_destroy(mpm_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graphs(b);
   if (b->dev) (void)hipFree(b->dev);
+  for (void *p : b->retired) (void)hipFree(p);
   for (hipEvent_t &e : b->ev)
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : b->gev) (void)hipEventDestroy(e);

@@ -189,6 +189,8 @@ void scene_dev(const mpm_scene *s, int phase, SceneDev *d) {
   d->ko.cols = s->tc.ppt;
   d->offsets = s->sort.offsets;
   d->src_of = s->sort.src_of;
+  d->spans = s->sort.spans;
+  d->so = src_out(s->sort, s->tc, d->D.density);
   d->ch = 256u * s->tc.ppt;
   d->keep = (s->cfg.flags & MPM_FLAG_KEEP_GRID) ? 1 : 0;
   d->W = s->walls;

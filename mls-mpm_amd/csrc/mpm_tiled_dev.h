@@ -982,7 +982,8 @@ __device__ __forceinline__ void write_items(const BrickWork &bw, uint32_t b, uin
 // The block size is a multiple of 64, at most 1024.
 template <int K>
 __device__ __forceinline__ void scan_items_block(const BrickWork &bw, uint32_t chs,
-                                                 uint32_t *counts, uint32_t *offsets) {
+                                                 uint32_t *counts, uint32_t *offsets,
+                                                 uint2 *spans = nullptr) {
   __shared__ uint32_t wp[16], wi[16];
   // items per size class (class k: R - k rows), then the next free dispatch
   // position of each class
@@ -1048,6 +1049,7 @@ __device__ __forceinline__ void scan_items_block(const BrickWork &bw, uint32_t c
       ei += ii - pc;
       if (r < nb) {
         offsets[r] = ep;
+        if (spans) spans[r] = make_uint2(ep, ep + c[k]);
         counts[r] = 0u;
         write_items(bw, bk[k], ep, c[k], chs, ei, cur, R);
       }
@@ -1060,21 +1062,5 @@ __device__ __forceinline__ void scan_items_block(const BrickWork &bw, uint32_t c
 
 
 
-// Inverse permutation of the re-sort: src_of[new slot] = old slot.
-__device__ __forceinline__ void build_src_one(uint32_t n, uint32_t nbricks, uint32_t ch,
-                                              const uint32_t *keys, const uint32_t *rank,
-                                              const uint32_t *offsets, uint32_t *src_of,
-                                              uint32_t s) {
-  if (s >= n) return;
-  uint32_t brick = keys[s];
-  uint32_t bstart = offsets[brick];
-  uint32_t bend = (brick + 1 < nbricks) ? offsets[brick + 1] : n;
-  uint32_t local = rank[s];
-  uint32_t piece = local / ch;
-  uint32_t istart = bstart + piece * ch;
-  uint32_t c = bend - istart;
-  if (c > ch) c = ch;
-  src_of[istart + item_slot(local - piece * ch, c)] = s;
-}
 
 }  // namespace mpm
