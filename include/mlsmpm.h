@@ -317,6 +317,10 @@ int mpm_tile_stats(mpm_scene *s, uint64_t out[4]);
  *     mpm_append_particles or mpm_delete_near) takes that step through the
  *     single-scene path, together with the other members.
  *   - mpm_destroy on a member removes it from its batch first.
+ *   - The batch's descriptor tables grow with its member set; a replaced
+ *     table buffer is freed by mpm_batch_destroy, not while the batch runs
+ *     (hipFree synchronizes the device, which fails a stream capture that
+ *     another thread has in progress).
  */
 typedef struct mpm_batch mpm_batch;
 
@@ -329,9 +333,9 @@ typedef struct mpm_batch mpm_batch;
 
 /* Launch shapes of the batched kernels, as chosen by the autotuner
  * (mlsmpm.autotune): persistent blocks of the work-item kernels (P2G mass,
- * P2G momentum, G2P), of the resolve kernels and of the per-brick /
- * per-particle kernels (0 = one block per unit), and steps per captured
- * graph (rounded up to even; 0 = 16). */
+ * P2G momentum, G2P), of the resolve kernels and of the per-brick kernel
+ * (the brick list; 0 = one block per unit), and steps per captured graph
+ * (rounded up to even; 0 = 16). */
 typedef struct mpm_batch_tuning {
     uint32_t item_blocks;
     uint32_t resolve_blocks;

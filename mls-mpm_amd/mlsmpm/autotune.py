@@ -148,7 +148,7 @@ def _time_batch(b: Batch, steps: int) -> float:
 
 def tune_batch(b: Batch, steps: int = 6, rounds: int = 2) -> capi.BatchTuning:
     """Coordinate descent over the persistent block counts of the work-item,
-    resolve and per-brick/per-particle kernel classes (timed eagerly, one HIP
+    resolve and per-brick kernel classes (timed eagerly, one HIP
     event pair per batched step).  Leaves the batch set to the result."""
     t = b.tuning()
     cur = {"item_blocks": t.item_blocks, "resolve_blocks": t.resolve_blocks,
